@@ -1,0 +1,484 @@
+// hfx_scf.cpp - host SCF driver over the engine ABI (include/hfx_scf.h).
+//
+// Host counterpart of the reference's SCF class, restated without Eigen:
+//   SCF::run ............................ src/SCF.cpp:23-136
+//   SCF::initialize ..................... src/SCF.cpp:138-162
+//   SCF::computeInitialGuessDensity ..... src/SCF.cpp:165-231
+//   incremental Fock update ............. src/SCF.cpp:536-541, 731-743 (J/K from the engine)
+//   SCF::diagonalizeAndUpdate ........... src/SCF.cpp:747-835
+//   DIIS update/extrapolate/B matrix .... src/DIIS.cpp:11-81
+//   inverseSqrtMatrix ................... src/Utils.cpp:117-138
+// Linear algebra (eigensolver, QR with column pivoting) comes from linalg.hpp.
+
+#include "../../include/hfx_scf.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "linalg.hpp"
+
+namespace {
+
+using Mat = std::vector<double>;
+using clk = std::chrono::steady_clock;
+
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+double trace_prod(int n, const double* A, const double* B) {  // tr(A B)
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) s += A[i + (size_t)k * n] * B[k + (size_t)i * n];
+    return s;
+}
+
+double frob(const Mat& A) {
+    double s = 0.0;
+    for (double x : A) s += x * x;
+    return std::sqrt(s);
+}
+
+// inverseSqrtMatrix (Utils.cpp:117-138): V diag(1/sqrt(w), 0 for w <= 0) V^T.
+Mat inv_sqrt(int n, const Mat& S) {
+    Mat V((size_t)n * n), w(n), T((size_t)n * n), X((size_t)n * n);
+    hfx_la::eigh(n, S.data(), w.data(), V.data());
+    for (int c = 0; c < n; ++c) {
+        const double f = w[c] > 0 ? 1.0 / std::sqrt(w[c]) : 0.0;
+        for (int r = 0; r < n; ++r) T[r + (size_t)c * n] = V[r + (size_t)c * n] * f;
+    }
+    hfx_la::mm(n, T.data(), V.data(), X.data(), false, true);
+    return X;
+}
+
+// DIIS (src/DIIS.cpp): FIFO of at most maxSize error vectors and Fock matrices.
+struct Diis {
+    size_t max_size;
+    std::deque<Mat> err, fa, fb;
+    explicit Diis(size_t m) : max_size(m) {}
+    void update(const Mat& Fa, const Mat& Fb, const Mat& e) {
+        if (err.size() >= max_size) {
+            err.pop_front();
+            fa.pop_front();
+            fb.pop_front();
+        }
+        err.push_back(e);
+        fa.push_back(Fa);
+        fb.push_back(Fb);
+    }
+    void extrapolate(Mat& Fa, Mat& Fb) const {
+        if (err.empty()) return;
+        const int m = (int)err.size() + 1;
+        Mat B((size_t)m * m, 0.0);
+        for (int i = 0; i + 1 < m; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double s = 0.0;
+                const Mat &ei = err[i], &ej = err[j];
+                for (size_t x = 0; x < ei.size(); ++x) s += ei[x] * ej[x];
+                B[i + (size_t)j * m] = B[j + (size_t)i * m] = s;
+            }
+        for (int i = 0; i < m; ++i) {
+            B[(m - 1) + (size_t)i * m] = -1.0;
+            B[i + (size_t)(m - 1) * m] = -1.0;
+        }
+        B[(m - 1) + (size_t)(m - 1) * m] = 0.0;
+        std::vector<double> b(m, 0.0);
+        b[m - 1] = -1.0;
+        const std::vector<double> c = hfx_la::lstsq_qrcp(m, m, B, b);
+        std::fill(Fa.begin(), Fa.end(), 0.0);
+        std::fill(Fb.begin(), Fb.end(), 0.0);
+        for (int i = 0; i + 1 < m; ++i) {
+            const Mat &ai = fa[i], &bi = fb[i];
+            for (size_t x = 0; x < Fa.size(); ++x) {
+                Fa[x] += c[i] * ai[x];
+                Fb[x] += c[i] * bi[x];
+            }
+        }
+    }
+    double err_norm() const { return err.empty() ? 0.0 : frob(err.back()); }
+};
+
+struct Scf {
+    hfx_engine* e = nullptr;
+    hfx_scf_options o{};
+    int n = 0, nS = 0, na = 0, nb = 0;
+    bool uhf = false;
+    double e_nuc = 0, e_elec = 0, dE = 0, dD = 0, diis_err = 0, damp_c = 0;
+    bool ls_a = false, ls_b = false;
+    int iteration = 0;
+    Mat S, X, h, Q;
+    Mat Fa, Fb, Fa_prev, Fb_prev, Ca, Cb, Da, Db, Da_prev, Db_prev, Dt, Dt_prev;
+    std::vector<double> eps_a, eps_b;
+    double t_jk = 0, t_diag = 0;
+    int64_t n_builds = 0, n_quart_last = 0;
+    int precision = HFX_FP64;
+
+    Mat mat() const { return Mat((size_t)n * n, 0.0); }
+
+    // X^T F X (+ level shift) -> eigenpairs -> C = X V (SCF.cpp:775-790).
+    void diag(const Mat& F, bool shift, const Mat& Cold, int nocc, Mat& C, std::vector<double>& w) {
+        const auto t0 = clk::now();
+        Mat T = mat(), Fp = mat();
+        hfx_la::mm(n, F.data(), X.data(), T.data());
+        hfx_la::mm(n, X.data(), T.data(), Fp.data(), true, false);
+        if (shift) {
+            // F' += (X S C) diag(shift on virtuals) (X S C)^T  (SCF.cpp:781-786)
+            Mat SC = mat(), XSC = mat(), XSCs = mat(), add = mat();
+            hfx_la::mm(n, S.data(), Cold.data(), SC.data());
+            hfx_la::mm(n, X.data(), SC.data(), XSC.data());
+            XSCs = XSC;
+            for (int c = 0; c < n; ++c)
+                for (int r = 0; r < n; ++r) XSCs[r + (size_t)c * n] *= (c >= nocc) ? o.level_shift : 0.0;
+            hfx_la::mm(n, XSCs.data(), XSC.data(), add.data(), false, true);
+            for (size_t x = 0; x < Fp.size(); ++x) Fp[x] += add[x];
+        }
+        Mat V = mat();
+        w.assign(n, 0.0);
+        hfx_la::eigh(n, Fp.data(), w.data(), V.data());
+        C = mat();
+        hfx_la::mm(n, X.data(), V.data(), C.data());
+        t_diag += secs(t0, clk::now());
+    }
+
+    // C_occ C_occ^T
+    Mat density(const Mat& C, int nocc) {
+        Mat D = mat();
+        if (nocc > 0) hfx_la::gemm(false, true, n, n, nocc, 1.0, C.data(), n, C.data(), n, 0.0, D.data(), n);
+        return D;
+    }
+
+    double energy() const {  // E_elec = 1/2 tr(Da (h + Fa)) + 1/2 tr(Db (h + Fb))  (SCF.cpp:833-834)
+        Mat ha((size_t)n * n), hb((size_t)n * n);
+        for (size_t x = 0; x < ha.size(); ++x) {
+            ha[x] = h[x] + Fa[x];
+            hb[x] = h[x] + Fb[x];
+        }
+        return 0.5 * trace_prod(n, Da.data(), ha.data()) + 0.5 * trace_prod(n, Db.data(), hb.data());
+    }
+
+    int build_fock(bool full) {  // SCF::buildFockMatrixDirect (SCF.cpp:523-744) via hfx_jk
+        if (full) {
+            std::fill(Dt_prev.begin(), Dt_prev.end(), 0.0);
+            std::fill(Da_prev.begin(), Da_prev.end(), 0.0);
+            std::fill(Db_prev.begin(), Db_prev.end(), 0.0);
+            Fa_prev = h;
+            Fb_prev = h;
+        }
+        Mat dDt = mat(), dDa = mat(), dDb;
+        for (size_t x = 0; x < dDt.size(); ++x) {
+            dDt[x] = Dt[x] - Dt_prev[x];
+            dDa[x] = Da[x] - Da_prev[x];
+        }
+        if (uhf) {
+            dDb = mat();
+            for (size_t x = 0; x < dDb.size(); ++x) dDb[x] = Db[x] - Db_prev[x];
+        }
+        Mat J = mat(), Ka = mat(), Kb;
+        if (uhf) Kb = mat();
+        hfx_jk_args a{};
+        a.dD_tot = dDt.data();
+        a.dD_alpha = dDa.data();
+        a.dD_beta = uhf ? dDb.data() : nullptr;
+        a.Q = Q.data();
+        a.schwarz_thresh = o.schwarz_thresh;
+        a.density_thresh = o.density_thresh;
+        a.precision = precision;
+        a.fp32_cut = o.fp32_cut;
+        a.flags = o.jk_flags;
+        hfx_jk_stats st{};
+        const auto t0 = clk::now();
+        const int rc = hfx_jk(e, &a, J.data(), Ka.data(), uhf ? Kb.data() : nullptr, &st);
+        t_jk += secs(t0, clk::now());
+        if (rc) return rc;
+        n_builds++;
+        n_quart_last = st.n_quartets;
+        for (size_t x = 0; x < J.size(); ++x) Fa[x] = Fa_prev[x] + J[x] - Ka[x];
+        Fa_prev = Fa;
+        if (uhf) {
+            for (size_t x = 0; x < J.size(); ++x) Fb[x] = Fb_prev[x] + J[x] - Kb[x];
+            Fb_prev = Fb;
+        } else {
+            Fb = Fa;
+            Fb_prev = Fa_prev;
+        }
+        return HFX_OK;
+    }
+
+    void diagonalize_and_update() {  // SCF.cpp:747-835
+        if (o.damp > 0 && o.max_damp_iter != 0 && o.max_damp_iter > iteration &&
+            (dE > o.stop_damp_thresh || iteration == 0))
+            damp_c = o.damp / 100.0;
+        else
+            damp_c = 0.0;
+        if (o.level_shift > 0 && iteration < o.max_lshift_iter && (dE > o.stop_lshift_thresh || iteration == 0)) {
+            const double ga = (na < n && na > 0) ? eps_a[na] - eps_a[na - 1] : std::numeric_limits<double>::infinity();
+            const double gb = (nb < n && nb > 0) ? eps_b[nb] - eps_b[nb - 1] : std::numeric_limits<double>::infinity();
+            ls_a = ga < o.lshift_gap_tol;
+            ls_b = gb < o.lshift_gap_tol;
+        } else {
+            ls_a = ls_b = false;
+        }
+        Mat Cnew;
+        diag(Fa, ls_a, Ca, na, Cnew, eps_a);
+        Ca = Cnew;
+        Da_prev = Da;
+        {
+            const Mat P = density(Ca, na);
+            for (size_t x = 0; x < Da.size(); ++x) Da[x] = (1 - damp_c) * P[x] + damp_c * Da[x];
+        }
+        Db_prev = Db;
+        if (uhf) {
+            diag(Fb, ls_b, Cb, nb, Cnew, eps_b);
+            Cb = Cnew;
+            const Mat P = density(Cb, nb);
+            for (size_t x = 0; x < Db.size(); ++x) Db[x] = (1 - damp_c) * P[x] + damp_c * Db[x];
+        } else {
+            Cb = Ca;
+            eps_b = eps_a;
+            Db = Da;
+        }
+        Dt_prev = Dt;
+        for (size_t x = 0; x < Dt.size(); ++x) Dt[x] = Da[x] + Db[x];
+        e_elec = energy();
+    }
+
+    void initial_guess() {  // SCF.cpp:165-231
+        Fa = h;
+        Fa_prev = Fa;
+        diag(Fa, false, Mat(), na, Ca, eps_a);
+        Da_prev = mat();
+        Da = density(Ca, na);
+        Db_prev = mat();
+        if (uhf) {
+            if (o.guess_mix > 0 && na >= 1 && na < n) {
+                const double k = o.guess_mix / 10.0;
+                const double f = 1.0 / std::sqrt(1 + k * k);
+                for (int r = 0; r < n; ++r) {
+                    const double homo = Ca[r + (size_t)(na - 1) * n], lumo = Ca[r + (size_t)na * n];
+                    Ca[r + (size_t)(na - 1) * n] = f * (homo + k * lumo);
+                    Ca[r + (size_t)na * n] = f * (lumo - k * homo);
+                }
+                Da = density(Ca, na);
+            }
+            Fb = h;
+            Fb_prev = Fb;
+            diag(Fb, false, Mat(), nb, Cb, eps_b);
+            Db = density(Cb, nb);
+        } else {
+            Fb = Fa;
+            Fb_prev = Fa_prev;
+            Cb = Ca;
+            eps_b = eps_a;
+            Db = Da;
+        }
+        Dt_prev = mat();
+        Dt = mat();
+        for (size_t x = 0; x < Dt.size(); ++x) Dt[x] = Da[x] + Db[x];
+        e_elec = energy();
+    }
+
+    // The SCF cycles of SCF::run (SCF.cpp:51-114); returns true when converged.
+    int cycles(Diis* diis, bool first_full, int& it_done, bool& conv, hfx_scf_result* res, int trace_base) {
+        conv = false;
+        for (iteration = 0; iteration < o.max_iter; ++iteration) {
+            const double last = e_elec;
+            const int rc = build_fock(o.full_builds || (first_full && iteration == 0));
+            if (rc) return rc;
+            if (diis) {
+                Mat t1 = mat(), t2 = mat(), ea = mat(), eb = mat();
+                hfx_la::mm(n, Fa.data(), Da.data(), t1.data());
+                hfx_la::mm(n, t1.data(), S.data(), t2.data());
+                hfx_la::mm(n, S.data(), Da.data(), t1.data());
+                hfx_la::mm(n, t1.data(), Fa.data(), ea.data());
+                for (size_t x = 0; x < ea.size(); ++x) ea[x] = t2[x] - ea[x];
+                hfx_la::mm(n, Fb.data(), Db.data(), t1.data());
+                hfx_la::mm(n, t1.data(), S.data(), t2.data());
+                hfx_la::mm(n, S.data(), Db.data(), t1.data());
+                hfx_la::mm(n, t1.data(), Fb.data(), eb.data());
+                for (size_t x = 0; x < eb.size(); ++x) eb[x] = t2[x] - eb[x] + ea[x];
+                diis->update(Fa, Fb, eb);
+                diis->extrapolate(Fa, Fb);
+            }
+            diagonalize_and_update();
+            dE = std::abs(e_elec - last);
+            {
+                double s = 0.0;
+                for (size_t x = 0; x < Dt.size(); ++x) {
+                    const double d = Dt[x] - Dt_prev[x];
+                    s += d * d;
+                }
+                dD = std::sqrt(s);
+            }
+            diis_err = diis ? diis->err_norm() : 0.0;
+            const int slot = trace_base + iteration;
+            if (res && res->trace && slot < res->trace_cap) {
+                res->trace[4 * slot + 0] = e_elec;
+                res->trace[4 * slot + 1] = dE;
+                res->trace[4 * slot + 2] = dD;
+                res->trace[4 * slot + 3] = diis_err;
+            }
+            if (dE < o.energy_tol && dD < o.density_tol && (!diis || diis_err < o.diis_error_tol)) {
+                conv = true;
+                it_done = iteration + 1;
+                return HFX_OK;
+            }
+        }
+        it_done = o.max_iter;
+        return HFX_OK;
+    }
+};
+
+thread_local std::string g_scf_err;
+
+}  // namespace
+
+extern "C" {
+
+void hfx_scf_default_options(hfx_scf_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iter = 30;
+    o->energy_tol = 1.0e-6;
+    o->density_tol = 1.0e-6;
+    o->use_diis = 1;
+    o->diis_size = 6;
+    o->diis_error_tol = 1.0e-6;
+    o->schwarz_thresh = 1.0e-10;
+    o->density_thresh = 1.0e-10;
+    o->unrestricted = -1;
+    o->guess_mix = 0;
+    o->damp = 0;
+    o->max_damp_iter = 30;
+    o->stop_damp_thresh = 0.0;
+    o->level_shift = 0.0;
+    o->lshift_gap_tol = std::numeric_limits<double>::infinity();
+    o->max_lshift_iter = 30;
+    o->stop_lshift_thresh = 0.0;
+    o->precision = HFX_FP64;
+    o->fp32_cut = 0.0;
+    o->final_fp64 = 1;
+    o->jk_flags = 0;
+    o->full_builds = 0;
+}
+
+int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz, int32_t charge,
+                       int32_t mult, const hfx_scf_options* opt, hfx_scf_result* res) {
+    if (!e || !res || n_atoms <= 0 || !Z || !xyz) return HFX_ERR_INVALID;
+    const auto t0 = clk::now();
+    Scf s;
+    s.e = e;
+    if (opt)
+        s.o = *opt;
+    else
+        hfx_scf_default_options(&s.o);
+    int64_t N = 0, nS = 0, nmol = 0, elems = 0;
+    int rc = hfx_sizes(e, &N, &nS, &nmol, &elems);
+    if (rc) return rc;
+    if (nmol != 1) return HFX_ERR_INVALID;
+    s.n = (int)N;
+    s.nS = (int)nS;
+    // Electron counts (SCF.cpp:140-142, Molecule.cpp:67-72).
+    long nel = -charge;
+    for (int i = 0; i < n_atoms; ++i) nel += (long)std::lround(Z[i]);
+    if (mult < 1 || nel < 0 || ((nel + mult - 1) % 2) != 0 || nel + 1 < mult) return HFX_ERR_INVALID;
+    s.na = (int)((nel + mult - 1) / 2);
+    s.nb = (int)((nel - mult + 1) / 2);
+    if (s.na > s.n) return HFX_ERR_INVALID;
+    s.uhf = s.o.unrestricted < 0 ? (mult != 1) : (s.o.unrestricted != 0);
+    s.precision = s.o.precision;
+
+    // SCF::initialize (SCF.cpp:138-162).
+    double en = 0.0;
+    for (int i = 0; i < n_atoms; ++i)
+        for (int j = i + 1; j < n_atoms; ++j) {
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                         dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            en += (Z[i] * Z[j]) / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    s.e_nuc = en;
+    Mat T = s.mat(), V = s.mat();
+    s.S = s.mat();
+    s.Q.assign((size_t)nS * nS, 0.0);
+    if ((rc = hfx_kinetic(e, T.data())) || (rc = hfx_nuclear(e, n_atoms, Z, xyz, nullptr, V.data())) ||
+        (rc = hfx_overlap(e, s.S.data())) || (rc = hfx_schwarz(e, s.Q.data())))
+        return rc;
+    s.h = s.mat();
+    for (size_t x = 0; x < s.h.size(); ++x) s.h[x] = T[x] + V[x];
+    s.X = inv_sqrt(s.n, s.S);
+    s.initial_guess();
+    const auto t_init = clk::now();
+
+    Diis diis_mixed(s.o.diis_size);
+    int its = 0;
+    bool conv = false;
+    rc = s.cycles(s.o.use_diis ? &diis_mixed : nullptr, false, its, conv, res, 0);
+    if (rc) return rc;
+    int total_its = its;
+    res->e_total_mixed = s.e_elec + s.e_nuc;
+    res->iterations_mixed = its;
+    if (s.precision == HFX_MIXED && s.o.final_fp64) {
+        // Restart the incremental sequence from a full FP64 build of the current density and
+        // iterate to convergence again in FP64.
+        s.precision = HFX_FP64;
+        Diis diis2(s.o.diis_size);
+        int its2 = 0;
+        rc = s.cycles(s.o.use_diis ? &diis2 : nullptr, true, its2, conv, res, its);
+        if (rc) return rc;
+        total_its += its2;
+    }
+    // Remove the level shift from the virtual eigenvalues (SCF.cpp:93-107).
+    if (s.ls_a)
+        for (int i = s.na; i < s.n; ++i) s.eps_a[i] -= s.o.level_shift;
+    if (s.ls_b)
+        for (int i = s.nb; i < s.n; ++i) s.eps_b[i] -= s.o.level_shift;
+
+    res->e_electronic = s.e_elec;
+    res->e_nuclear = s.e_nuc;
+    res->e_total = s.e_elec + s.e_nuc;
+    res->iterations = total_its;
+    res->converged = conv ? 1 : 0;
+    res->last_dE = s.dE;
+    res->last_dD = s.dD;
+    res->last_diis_err = s.diis_err;
+    res->time_init_s = secs(t0, t_init);
+    res->time_jk_s = s.t_jk;
+    res->time_diag_s = s.t_diag;
+    res->n_jk_builds = s.n_builds;
+    res->n_quartets_last = s.n_quart_last;
+    auto put = [&](double* dst, const Mat& src) {
+        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
+    };
+    put(res->F_alpha, s.Fa);
+    put(res->D_alpha, s.Da);
+    put(res->C_alpha, s.Ca);
+    put(res->F_beta, s.Fb);
+    put(res->D_beta, s.Db);
+    put(res->C_beta, s.Cb);
+    put(res->h_core, s.h);
+    put(res->S, s.S);
+    if (res->eps_alpha) std::memcpy(res->eps_alpha, s.eps_a.data(), s.n * sizeof(double));
+    if (res->eps_beta) std::memcpy(res->eps_beta, s.eps_b.data(), s.n * sizeof(double));
+    res->time_total_s = secs(t0, clk::now());
+    return HFX_OK;
+}
+
+int hfx_scf_run(const hfx_basis* basis, int32_t n_atoms, const double* Z, const double* xyz, int32_t charge,
+                int32_t mult, const hfx_scf_options* opt, int device, hfx_scf_result* res) {
+    hfx_engine* e = nullptr;
+    const auto t0 = clk::now();
+    int rc = hfx_create(basis, device, &e);
+    if (rc) return rc;
+    const double t_create = secs(t0, clk::now());
+    rc = hfx_scf_run_engine(e, n_atoms, Z, xyz, charge, mult, opt, res);
+    hfx_destroy(e);
+    if (rc == HFX_OK) {
+        res->time_init_s += t_create;
+        res->time_total_s += t_create;
+    }
+    return rc;
+}
+
+}  // extern "C"

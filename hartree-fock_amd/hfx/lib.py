@@ -1,0 +1,301 @@
+"""ctypes binding of the engine ABI (include/hfx.h) and the SCF driver (include/hfx_scf.h).
+
+``Library(path)`` binds any shared library that exports the hfx.h symbols.  ``load()``
+returns the HIP engine built in this package (``lib/libhfx_mi355x.so``) and raises if it is
+missing: there is no CPU fallback behind the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libhfx_mi355x.so")
+
+HFX_OK = 0
+HFX_FP64 = 0
+HFX_MIXED = 1
+HFX_JK_DEVICE_PTRS = 1
+HFX_JK_USE_GRAPH = 2
+HFX_JK_NO_STATS = 4
+N_CLASSES = 21
+
+ERRORS = {1: "invalid argument", 2: "unsupported", 3: "out of memory", 4: "device error",
+          5: "non-finite input", 6: "not converged"}
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class HfxBasis(C.Structure):
+    _fields_ = [("n_shells", C.c_int32), ("n_prims", C.c_int32), ("n_aos", C.c_int32), ("n_norm", C.c_int32),
+                ("l", _i32p), ("n_prim", _i32p), ("prim_offset", _i32p), ("ao_offset", _i32p),
+                ("norm_offset", _i32p), ("centers", _f64p), ("alpha", _f64p), ("coeff", _f64p),
+                ("norm", _f64p), ("cart", _i32p)]
+
+
+class HfxJkArgs(C.Structure):
+    _fields_ = [("dD_tot", _f64p), ("dD_alpha", _f64p), ("dD_beta", _f64p), ("Q", _f64p),
+                ("schwarz_thresh", C.c_double), ("density_thresh", C.c_double), ("precision", C.c_int32),
+                ("fp32_cut", C.c_double), ("flags", C.c_int32)]
+
+
+class HfxJkStats(C.Structure):
+    _fields_ = [("n_quartets", C.c_int64), ("n_fp32", C.c_int64), ("class_count", C.c_int64 * N_CLASSES),
+                ("hash_sum", C.c_uint64), ("hash_xor", C.c_uint64), ("model_gflop", C.c_double),
+                ("device_ms", C.c_double)]
+
+    def as_dict(self):
+        return dict(n_quartets=self.n_quartets, n_fp32=self.n_fp32, class_count=list(self.class_count),
+                    hash_sum=self.hash_sum, hash_xor=self.hash_xor, model_gflop=self.model_gflop,
+                    device_ms=self.device_ms)
+
+
+class HfxScfOptions(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("density_tol", C.c_double),
+                ("use_diis", C.c_int32), ("diis_size", C.c_int32), ("diis_error_tol", C.c_double),
+                ("schwarz_thresh", C.c_double), ("density_thresh", C.c_double), ("unrestricted", C.c_int32),
+                ("guess_mix", C.c_int32), ("damp", C.c_int32), ("max_damp_iter", C.c_int32),
+                ("stop_damp_thresh", C.c_double), ("level_shift", C.c_double), ("lshift_gap_tol", C.c_double),
+                ("max_lshift_iter", C.c_int32), ("stop_lshift_thresh", C.c_double), ("precision", C.c_int32),
+                ("fp32_cut", C.c_double), ("final_fp64", C.c_int32), ("jk_flags", C.c_int32),
+                ("full_builds", C.c_int32)]
+
+
+class HfxScfResult(C.Structure):
+    _fields_ = [("e_total", C.c_double), ("e_electronic", C.c_double), ("e_nuclear", C.c_double),
+                ("iterations", C.c_int32), ("converged", C.c_int32), ("last_dE", C.c_double),
+                ("last_dD", C.c_double), ("last_diis_err", C.c_double), ("e_total_mixed", C.c_double),
+                ("iterations_mixed", C.c_int32), ("time_total_s", C.c_double), ("time_init_s", C.c_double),
+                ("time_jk_s", C.c_double), ("time_diag_s", C.c_double), ("n_jk_builds", C.c_int64),
+                ("n_quartets_last", C.c_int64),
+                ("F_alpha", _f64p), ("D_alpha", _f64p), ("C_alpha", _f64p), ("eps_alpha", _f64p),
+                ("F_beta", _f64p), ("D_beta", _f64p), ("C_beta", _f64p), ("eps_beta", _f64p),
+                ("h_core", _f64p), ("S", _f64p), ("trace", _f64p), ("trace_cap", C.c_int32)]
+
+
+ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
+               "hfx_nuclear", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk", "hfx_last_error",
+               "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine"]
+
+
+def _p(a, t=_f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class HfxError(RuntimeError):
+    pass
+
+
+class Library:
+    """A loaded library exporting the hfx.h ABI."""
+
+    def __init__(self, path):
+        if not os.path.exists(path):
+            raise HfxError("engine library not found: %s (build it first)" % path)
+        self.path = path
+        self.dll = C.CDLL(path)
+        d = self.dll
+        d.hfx_last_error.restype = C.c_char_p
+        d.hfx_create.argtypes = [C.POINTER(HfxBasis), C.c_int, C.POINTER(C.c_void_p)]
+        d.hfx_create_batched.argtypes = [C.c_int32, C.POINTER(HfxBasis), C.c_int, C.POINTER(C.c_void_p)]
+        d.hfx_destroy.argtypes = [C.c_void_p]
+        d.hfx_sizes.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 4
+        for f in ("hfx_overlap", "hfx_kinetic", "hfx_schwarz"):
+            getattr(d, f).argtypes = [C.c_void_p, _f64p]
+        d.hfx_nuclear.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _i32p, _f64p]
+        d.hfx_eri_quartets.argtypes = [C.c_void_p, C.c_int64, _i32p, C.POINTER(C.c_int64), _f64p]
+        d.hfx_jk.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.POINTER(HfxJkStats)]
+        d.hfx_scf_default_options.argtypes = [C.POINTER(HfxScfOptions)]
+        d.hfx_scf_default_options.restype = None
+        d.hfx_scf_run.argtypes = [C.POINTER(HfxBasis), C.c_int32, _f64p, _f64p, C.c_int32, C.c_int32,
+                                  C.POINTER(HfxScfOptions), C.c_int, C.POINTER(HfxScfResult)]
+        d.hfx_scf_run_engine.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, C.c_int32, C.c_int32,
+                                         C.POINTER(HfxScfOptions), C.POINTER(HfxScfResult)]
+
+    def check(self, rc, what):
+        if rc != HFX_OK:
+            msg = self.dll.hfx_last_error()
+            raise HfxError("%s failed (%s): %s" % (what, ERRORS.get(rc, rc), msg.decode() if msg else ""))
+
+    def default_options(self, **kw):
+        o = HfxScfOptions()
+        self.dll.hfx_scf_default_options(C.byref(o))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        return o
+
+
+class _BasisHolder:
+    """Keeps the numpy arrays behind an HfxBasis struct alive."""
+
+    def __init__(self, basis):
+        f = basis.flat()
+        self.arrays = f
+        b = HfxBasis()
+        b.n_shells, b.n_prims, b.n_aos, b.n_norm = f["n_shells"], f["n_prims"], f["n_aos"], f["n_norm"]
+        for k in ("l", "n_prim", "prim_offset", "ao_offset", "norm_offset", "cart"):
+            setattr(b, k, _p(f[k], _i32p))
+        for k in ("centers", "alpha", "coeff", "norm"):
+            setattr(b, k, _p(f[k]))
+        self.struct = b
+
+
+class Engine:
+    """One engine handle (single molecule, or a batch of molecules)."""
+
+    def __init__(self, library, basis, device=0):
+        self.lib = library
+        bases = basis if isinstance(basis, (list, tuple)) else [basis]
+        self.bases = list(bases)
+        self._holders = [_BasisHolder(b) for b in self.bases]
+        h = C.c_void_p()
+        if len(self.bases) == 1 and not isinstance(basis, (list, tuple)):
+            rc = library.dll.hfx_create(C.byref(self._holders[0].struct), device, C.byref(h))
+        else:
+            arr = (HfxBasis * len(self.bases))(*[x.struct for x in self._holders])
+            rc = library.dll.hfx_create_batched(len(self.bases), arr, device, C.byref(h))
+        library.check(rc, "hfx_create")
+        self.h = h
+        n = [C.c_int64() for _ in range(4)]
+        library.check(library.dll.hfx_sizes(self.h, *[C.byref(x) for x in n]), "hfx_sizes")
+        self.n_aos, self.n_shells, self.n_mol, self.n_elems = [x.value for x in n]
+        self.sizes = [b.n_aos for b in self.bases]
+        self.shell_counts = [b.n_shells for b in self.bases]
+
+    def close(self):
+        if self.h:
+            self.lib.dll.hfx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # packed N_m x N_m blocks <-> list of matrices
+    def unpack(self, flat):
+        out, off = [], 0
+        for n in self.sizes:
+            out.append(flat[off:off + n * n].reshape(n, n, order="F"))
+            off += n * n
+        return out if len(out) > 1 else out[0]
+
+    def pack(self, mats):
+        if isinstance(mats, np.ndarray) and mats.ndim == 2:
+            mats = [mats]
+        return np.ascontiguousarray(np.concatenate([np.asarray(m, np.float64).reshape(-1, order="F") for m in mats]))
+
+    def _mat(self, fn):
+        out = np.zeros(self.n_elems)
+        self.lib.check(getattr(self.lib.dll, fn)(self.h, _p(out)), fn)
+        return self.unpack(out)
+
+    def overlap(self):
+        return self._mat("hfx_overlap")
+
+    def kinetic(self):
+        return self._mat("hfx_kinetic")
+
+    def nuclear(self, Z, xyz, atom_mol=None):
+        Z = np.ascontiguousarray(np.asarray(Z, np.float64))
+        xyz = np.ascontiguousarray(np.asarray(xyz, np.float64).reshape(-1))
+        am = None if atom_mol is None else np.ascontiguousarray(np.asarray(atom_mol, np.int32))
+        out = np.zeros(self.n_elems)
+        self.lib.check(self.lib.dll.hfx_nuclear(self.h, len(Z), _p(Z), _p(xyz), _p(am, _i32p), _p(out)),
+                       "hfx_nuclear")
+        return self.unpack(out)
+
+    def schwarz(self):
+        n = sum(s * s for s in self.shell_counts)
+        out = np.zeros(n)
+        self.lib.check(self.lib.dll.hfx_schwarz(self.h, _p(out)), "hfx_schwarz")
+        res, off = [], 0
+        for s in self.shell_counts:
+            res.append(out[off:off + s * s].reshape(s, s, order="F"))
+            off += s * s
+        return res if len(res) > 1 else res[0]
+
+    def eri_quartets(self, quartets):
+        """Reference-layout blocks of contracted ERIs for a list of (a,b,c,d) shell quartets."""
+        q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+        sh = self.bases[0].shells
+        sizes = [sh[a].nao * sh[b].nao * sh[c].nao * sh[d].nao for a, b, c, d in q]
+        offs = np.zeros(len(q), np.int64)
+        if len(q):
+            offs[1:] = np.cumsum(sizes)[:-1]
+        out = np.zeros(int(sum(sizes)))
+        self.lib.check(self.lib.dll.hfx_eri_quartets(self.h, len(q), _p(q, _i32p), _p(offs, C.POINTER(C.c_int64)),
+                                                     _p(out)), "hfx_eri_quartets")
+        res = []
+        for (a, b, c, d), o, s in zip(q, offs, sizes):
+            res.append(out[o:o + s].reshape(sh[a].nao, sh[b].nao, sh[c].nao, sh[d].nao))
+        return res
+
+    def jk(self, dD_tot, dD_alpha, dD_beta=None, Q=None, schwarz_thresh=1e-10, density_thresh=1e-10,
+           precision=HFX_FP64, fp32_cut=0.0, flags=0):
+        dt = self.pack(dD_tot)
+        da = self.pack(dD_alpha)
+        db = None if dD_beta is None else self.pack(dD_beta)
+        q = None
+        if Q is not None:
+            qs = Q if isinstance(Q, (list, tuple)) else [Q]
+            q = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1, order="F") for x in qs]))
+        a = HfxJkArgs()
+        a.dD_tot, a.dD_alpha, a.dD_beta, a.Q = _p(dt), _p(da), _p(db), _p(q)
+        a.schwarz_thresh, a.density_thresh = schwarz_thresh, density_thresh
+        a.precision, a.fp32_cut, a.flags = precision, fp32_cut, flags
+        J = np.zeros(self.n_elems)
+        Ka = np.zeros(self.n_elems)
+        Kb = np.zeros(self.n_elems) if db is not None else None
+        st = HfxJkStats()
+        self.lib.check(self.lib.dll.hfx_jk(self.h, C.byref(a), _p(J), _p(Ka), _p(Kb), C.byref(st)), "hfx_jk")
+        return self.unpack(J), self.unpack(Ka), (None if Kb is None else self.unpack(Kb)), st.as_dict()
+
+
+def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, outputs=True, trace_cap=200):
+    """Run the library's SCF driver; returns a dict with energies and (optionally) matrices."""
+    o = options if options is not None else library.default_options()
+    r = HfxScfResult()
+    n = basis.n_aos
+    keep = {}
+    if outputs:
+        for k in ("F_alpha", "D_alpha", "C_alpha", "F_beta", "D_beta", "C_beta", "h_core", "S"):
+            keep[k] = np.zeros(n * n)
+            setattr(r, k, _p(keep[k]))
+        for k in ("eps_alpha", "eps_beta"):
+            keep[k] = np.zeros(n)
+            setattr(r, k, _p(keep[k]))
+    keep["trace"] = np.zeros(4 * trace_cap)
+    r.trace = _p(keep["trace"])
+    r.trace_cap = trace_cap
+    hb = _BasisHolder(basis)
+    Z = np.ascontiguousarray(np.asarray(basis.Z, np.float64))
+    xyz = np.ascontiguousarray(basis.xyz.reshape(-1))
+    rc = library.dll.hfx_scf_run(C.byref(hb.struct), len(Z), _p(Z), _p(xyz), charge, multiplicity, C.byref(o),
+                                 device, C.byref(r))
+    library.check(rc, "hfx_scf_run")
+    out = {k: getattr(r, k) for k, _ in HfxScfResult._fields_ if not k.startswith(("F_", "D_", "C_", "eps_", "h_", "S", "trace"))}
+    it = r.iterations
+    out["trace"] = keep["trace"][:4 * min(it, trace_cap)].reshape(-1, 4)
+    for k, v in keep.items():
+        if k == "trace":
+            continue
+        out[k] = v.reshape(n, n, order="F") if v.size == n * n else v
+    return out
+
+
+_LIB = None
+
+
+def load(path=None):
+    """The HIP engine library.  Raises HfxError when it has not been built."""
+    global _LIB
+    if path is not None:
+        return Library(path)
+    if _LIB is None:
+        _LIB = Library(LIB_PATH)
+    return _LIB

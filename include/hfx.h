@@ -1,0 +1,151 @@
+/*
+ * hfx.h - C ABI of the MI355X direct-SCF Fock-build engine.
+ *
+ * The engine replaces the reference's C++ class IntegralEngine (include/IntegralEngine.hpp:8-35)
+ * and the shell-quartet loop of SCF::buildFockMatrixDirect (src/SCF.cpp:523-744) with one call
+ * per Fock build.  Everything crossing this boundary is a plain pointer plus sizes; no C++ or
+ * torch types, no exceptions.  Every function returns an int status (HFX_OK = 0); on failure
+ * hfx_last_error() returns a thread-local message.
+ *
+ * Matrices are N x N, column-major (element (r, c) at r + c*N), full and symmetric, as the
+ * reference's Eigen::MatrixXd after selfadjointView<Upper> (src/SCF.cpp:727-728).
+ * The caller owns every host array for the duration of a call; the engine keeps no pointer.
+ * Calls on one engine are serialized on that engine's HIP stream; different engines are
+ * independent.
+ */
+#ifndef HFX_H
+#define HFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HFX_OK = 0,
+    HFX_ERR_INVALID = 1,     /* bad handle, NULL pointer, inconsistent sizes            */
+    HFX_ERR_UNSUPPORTED = 2, /* angular momentum above d, unexpected Cartesian order    */
+    HFX_ERR_NOMEM = 3,       /* device or host allocation failed                        */
+    HFX_ERR_DEVICE = 4,      /* a HIP call failed                                       */
+    HFX_ERR_NONFINITE = 5,   /* NaN or Inf in an input matrix                           */
+    HFX_ERR_NOT_CONVERGED = 6
+};
+
+enum { HFX_FP64 = 0, HFX_MIXED = 1 };
+
+/* hfx_jk_args.flags */
+enum {
+    HFX_JK_DEVICE_PTRS = 1, /* density inputs and J/K outputs are device pointers           */
+    HFX_JK_USE_GRAPH = 2,   /* replay the engine's captured hipGraph of the build pipeline   */
+    HFX_JK_NO_STATS = 4     /* skip the survivor statistics read-back                         */
+};
+
+#define HFX_MAX_L 2
+#define HFX_N_CLASSES 21 /* quartet classes (la>=lb, lc>=ld, (la,lb)>=(lc,ld)) for l <= 2 */
+
+/* Flat basis layout: the struct-of-arrays of IntegralEngine::IntegralEngine
+ * (src/IntegralEngine.cpp:13-75, fields include/IntegralEngine.hpp:145-168).
+ * Shell order, Cartesian order and per-primitive per-component normalization are those of
+ * BasisSet::BasisSet (src/Basis.cpp:14-86). */
+typedef struct hfx_basis {
+    int32_t n_shells;
+    int32_t n_prims;            /* sum of n_prim                                       */
+    int32_t n_aos;              /* sum of (l+1)(l+2)/2                                 */
+    int32_t n_norm;             /* sum of n_prim * n_ao                                */
+    const int32_t* l;           /* [n_shells]  lShell                                  */
+    const int32_t* n_prim;      /* [n_shells]  nPrimitives                             */
+    const int32_t* prim_offset; /* [n_shells]  primitiveOffsets                        */
+    const int32_t* ao_offset;   /* [n_shells]  aoOffsets                               */
+    const int32_t* norm_offset; /* [n_shells]  normFactorOffsets                       */
+    const double* centers;      /* [3*n_shells] x,y,z of each shell (3 x nS, col-major) */
+    const double* alpha;        /* [n_prims]   exponents                               */
+    const double* coeff;        /* [n_prims]   contraction coefficients                */
+    const double* norm;         /* [n_norm]    norm[norm_offset[s] + p*n_ao + comp]    */
+    const int32_t* cart;        /* [3*n_aos]   (i,j,k) of each AO                      */
+} hfx_basis;
+
+typedef struct hfx_engine hfx_engine;
+
+/* Arguments of one J/K build.  Replaces the members SCF::buildFockMatrixDirect reads:
+ * D_tot - D_tot_prev, D_alpha - D_alpha_prev, D_beta - D_beta_prev (src/SCF.cpp:536-541),
+ * Q (src/SCF.cpp:605), options.schwartzThreshold / densityThreshold (src/SCF.cpp:605,622). */
+typedef struct hfx_jk_args {
+    const double* dD_tot;   /* N x N                                                       */
+    const double* dD_alpha; /* N x N                                                       */
+    const double* dD_beta;  /* N x N, or NULL for RHF (no K_beta)                          */
+    const double* Q;        /* nS x nS Schwarz table, or NULL for the engine's own         */
+    double schwarz_thresh;  /* tau_S: shell test Q(a,b)Q(c,d) < tau_S and AO test |eri|<tau_S */
+    double density_thresh;  /* tau_D                                                       */
+    int32_t precision;      /* HFX_FP64 or HFX_MIXED                                       */
+    double fp32_cut;        /* MIXED: quartets with Q(ab)Q(cd)max|dD| < fp32_cut run in FP32 */
+    int32_t flags;          /* HFX_JK_*                                                    */
+} hfx_jk_args;
+
+/* Survivor statistics of one build.  The list hash is an order-independent multiset hash of
+ * the surviving shell quartets written in the reference's canonical form (a>=b, c>=d,
+ * a(a+1)/2+b >= c(c+1)/2+d; src/SCF.cpp:602-603), so two builds have the same work list
+ * exactly when counts and both hashes agree. */
+typedef struct hfx_jk_stats {
+    int64_t n_quartets;                   /* surviving shell quartets                     */
+    int64_t n_fp32;                       /* of which computed in FP32 (MIXED)            */
+    int64_t class_count[HFX_N_CLASSES];   /* survivors per angular-momentum class         */
+    uint64_t hash_sum;                    /* sum of splitmix64(((a*nS+b)*nS+c)*nS+d)      */
+    uint64_t hash_xor;                    /* xor of the same                              */
+    double model_gflop;                   /* algorithmic GFLOP of the surviving work      */
+    double device_ms;                     /* device time of the build (HIP events)        */
+} hfx_jk_stats;
+
+/* IntegralEngine::IntegralEngine (src/IntegralEngine.cpp:13-75).  Builds the device basis,
+ * primitive-pair data, the Boys table and the engine's Schwarz table. */
+int hfx_create(const hfx_basis* basis, int device, hfx_engine** out);
+
+/* Many independent molecules behind one engine: quartets never mix molecules, and every
+ * N x N argument or result becomes the concatenation of the molecules' N_m x N_m
+ * column-major blocks, in molecule order.  No reference counterpart (the reference runs one
+ * molecule per process). */
+int hfx_create_batched(int32_t n_mol, const hfx_basis* bases, int device, hfx_engine** out);
+
+int hfx_destroy(hfx_engine* e);
+
+/* Sizes of the engine's layout: AOs, shells, molecules, and the packed N x N element count. */
+int hfx_sizes(const hfx_engine* e, int64_t* n_aos, int64_t* n_shells, int64_t* n_mol,
+              int64_t* n_matrix_elems);
+
+/* IntegralEngine::overlapMatrix (src/IntegralEngine.cpp:145-159). */
+int hfx_overlap(hfx_engine* e, double* S);
+/* IntegralEngine::kineticMatrix (src/IntegralEngine.cpp:241-256). */
+int hfx_kinetic(hfx_engine* e, double* T);
+/* IntegralEngine::nuclearAttractionMatrix (src/IntegralEngine.cpp:355-368); atoms are the
+ * point charges Z at xyz[3*i..3*i+2] (Atom in include/Utils.hpp).  Batched engines take the
+ * atoms of every molecule concatenated, with atom_mol[i] naming the molecule of atom i
+ * (NULL for single-molecule engines). */
+int hfx_nuclear(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz,
+                const int32_t* atom_mol, double* V);
+
+/* IntegralEngine::schwartzScreeningMatrix (src/IntegralEngine.cpp:695-728): nS x nS. */
+int hfx_schwarz(hfx_engine* e, double* Q);
+
+/* IntegralEngine::electronRepulsion(a,b,c,d, std::vector<double>&)
+ * (src/IntegralEngine.cpp:731-992), for n quartets at once: quartet q is shells
+ * abcd[4q..4q+3]; its naoA*naoB*naoC*naoD values go to out[offsets[q]...] in row-major
+ * (a,b,c,d) order, with the entries the reference leaves at zero set to zero
+ * (aoB > aoA, aoD > aoC, and (A==C && B==D && I < K); src/IntegralEngine.cpp:857-882). */
+int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_t* offsets,
+                     double* out);
+
+/* One direct J/K build: the work of src/SCF.cpp:536-728 (density maxima, screened quartet
+ * loop, digestion, reduction, symmetrization).  Outputs the increments J[dD_tot],
+ * K_alpha[dD_alpha] and, when dD_beta is given, K_beta[dD_beta]; the caller forms
+ * F = F_prev + J - K (src/SCF.cpp:731).  Kb may be NULL for RHF.  stats may be NULL. */
+int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb,
+           hfx_jk_stats* stats);
+
+/* Thread-local description of the last error on this thread. */
+const char* hfx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_H */

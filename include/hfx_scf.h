@@ -1,0 +1,89 @@
+/*
+ * hfx_scf.h - host SCF driver over the hfx.h engine ABI.
+ *
+ * The reference's SCF class (include/SCF.hpp, src/SCF.cpp) owns the iteration loop, the
+ * core-Hamiltonian guess, DIIS, damping and level shifting, and calls the integral engine
+ * for the one-electron matrices, the Schwarz table and, every iteration, the direct Fock
+ * build.  This driver is the host counterpart of that class, written against hfx.h only: it
+ * runs unchanged over the HIP engine or over any other library exporting hfx.h.
+ */
+#ifndef HFX_SCF_H
+#define HFX_SCF_H
+
+#include <stdint.h>
+
+#include "hfx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* SCFOptions (include/SCF.hpp:9-32) plus the engine's precision controls. */
+typedef struct hfx_scf_options {
+    int32_t max_iter;          /* maxIter            30      */
+    double energy_tol;         /* energyTol          1e-6    */
+    double density_tol;        /* densityTol         1e-6    */
+    int32_t use_diis;          /* useDIIS            1       */
+    int32_t diis_size;         /* DIISmaxSize        6       */
+    double diis_error_tol;     /* DIISErrorTol       1e-6    */
+    double schwarz_thresh;     /* schwartzThreshold  1e-10   */
+    double density_thresh;     /* densityThreshold   1e-10   */
+    int32_t unrestricted;      /* -1: multiplicity != 1 (Input.cpp:127-130), 0 RHF, 1 UHF */
+    int32_t guess_mix;         /* guessMix (0..10)   0       */
+    int32_t damp;              /* damp, percent      0       */
+    int32_t max_damp_iter;     /* maxDampIter        30      */
+    double stop_damp_thresh;   /* stopDampThresh     0       */
+    double level_shift;        /* levelShift         0       */
+    double lshift_gap_tol;     /* lshiftGapTol       +inf    */
+    int32_t max_lshift_iter;   /* maxLshiftIter      30      */
+    double stop_lshift_thresh; /* stopLshiftThresh   0       */
+    int32_t precision;         /* HFX_FP64 or HFX_MIXED for the incremental builds          */
+    double fp32_cut;           /* MIXED: FP32 below this quartet bound                      */
+    int32_t final_fp64;        /* MIXED: after convergence, restart from a full FP64 build  */
+                               /* and iterate to convergence again                         */
+    int32_t jk_flags;          /* extra HFX_JK_* flags for every build                      */
+    int32_t full_builds;       /* 1: every build from the full density (no increments)      */
+} hfx_scf_options;
+
+typedef struct hfx_scf_result {
+    double e_total, e_electronic, e_nuclear;
+    int32_t iterations, converged;
+    double last_dE, last_dD, last_diis_err;
+    double e_total_mixed;        /* MIXED + final_fp64: energy at the mixed-precision stop */
+    int32_t iterations_mixed;
+    double time_total_s, time_init_s, time_jk_s, time_diag_s;
+    int64_t n_jk_builds;
+    int64_t n_quartets_last;     /* survivors of the last build                             */
+    /* Optional outputs, caller-allocated (N x N or N), may be NULL. */
+    double* F_alpha;
+    double* D_alpha;
+    double* C_alpha;
+    double* eps_alpha;
+    double* F_beta;
+    double* D_beta;
+    double* C_beta;
+    double* eps_beta;
+    double* h_core;
+    double* S;
+    /* Optional per-iteration trace: trace[4*it + {0,1,2,3}] = E_elec, dE, dD, DIIS error. */
+    double* trace;
+    int32_t trace_cap;           /* number of iterations the trace can hold                 */
+} hfx_scf_result;
+
+void hfx_scf_default_options(hfx_scf_options* opt);
+
+/* SCF::run (src/SCF.cpp:23-136) on the molecule (Z, xyz in Bohr) in the given basis. */
+int hfx_scf_run(const hfx_basis* basis, int32_t n_atoms, const double* Z, const double* xyz,
+                int32_t charge, int32_t multiplicity, const hfx_scf_options* opt, int device,
+                hfx_scf_result* res);
+
+/* The same loop over an engine the caller created (the engine is not destroyed). */
+int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz,
+                       int32_t charge, int32_t multiplicity, const hfx_scf_options* opt,
+                       hfx_scf_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_SCF_H */
