@@ -1,0 +1,209 @@
+"""Benchmark: direct-SCF Fock builds/s and SCF wall time on one MI355X, FP64 vs mixed precision.
+
+Workload: linear all-trans C20H42 in cc-pVDZ with its general contractions expanded (510
+Cartesian functions, 246 shells), seeded +-0.02 Bohr jitter.  One step = one full Fock
+build (one hfx_jk call, dD = D_conv, D_prev = 0) from device-resident inputs.  The run:
+  1. FP64 SCF (Schwarz 1e-10, density 1e-10, DIIS 6, energy tolerance 1e-8) -> D_conv, E64
+  2. autotuned FP32 cut at D_conv, then the mixed-precision SCF with a final FP64 restart
+  3. W warm-up + K timed FP64 builds, then W + K timed mixed builds; HIP events time every
+     class kernel group inside the timed builds (roofline of the dominant one)
+  4. CPU baseline: the oracle's OpenMP port (reference release flags) on a bounded slice
+     of the same build, scaled by the model FLOPs of the slice
+Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hartree-fock_amd"))
+
+import numpy as np  # noqa: E402
+
+from hfx import Basis, autotune, device, geometry, lib  # noqa: E402
+
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X: FP64 vector / dense MFMA; FP32 dense MFMA
+CLASS_NAMES = ["ssss", "psss", "psps", "ppss", "ppps", "pppp", "dsss", "dsps", "dspp", "dsds", "dpss", "dpps",
+               "dppp", "dpds", "dpdp", "ddss", "ddps", "ddpp", "ddds", "dddp", "dddd"]
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def timed_builds(L, eng, bufs, n, steps, warmup, precision, cut, Q):
+    dDt, dDa, J, Ka = bufs
+    kw = dict(Q=Q, precision=precision, fp32_cut=cut, flags=lib.HFX_JK_CLASS_TIMING)
+    for _ in range(warmup):
+        lib.jk_device(eng, dDt, dDa, J, Ka, **kw)
+    device.synchronize()
+    ms64 = np.zeros(21)
+    ms32 = np.zeros(21)
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(steps):
+        st = lib.jk_device(eng, dDt, dDa, J, Ka, **kw)
+        ms64 += st["class_ms64"]
+        ms32 += st["class_ms32"]
+    device.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, st, ms64 / steps, ms32 / steps
+
+
+def cpu_baseline(basis, Da, Q, total_gflop, budget_s):
+    """Oracle port (OpenMP, reference release flags) on bra shells [a0, a1) of the same build."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import pyoracle
+    O = pyoracle.load(fast=True)
+    e = lib.Engine(O, basis)
+    n = basis.n_aos
+    nS = basis.n_shells
+    Dt = np.ascontiguousarray((2 * Da).reshape(-1, order="F"))
+    Dal = np.ascontiguousarray(Da.reshape(-1, order="F"))
+    Qf = np.ascontiguousarray(Q.reshape(-1, order="F"))
+    f64p = C.POINTER(C.c_double)
+    a = lib.HfxJkArgs()
+    a.dD_tot, a.dD_alpha, a.dD_beta = Dt.ctypes.data_as(f64p), Dal.ctypes.data_as(f64p), None
+    a.Q = Qf.ctypes.data_as(f64p)
+    a.schwarz_thresh, a.density_thresh, a.precision = 1e-10, 1e-10, lib.HFX_FP64
+    J = np.zeros(n * n)
+    K = np.zeros(n * n)
+
+    def run(a0, a1):
+        st = lib.HfxJkStats()
+        t = time.perf_counter()
+        rc = O.dll.oracle_jk_range(e.h, C.byref(a), a0, a1, J.ctypes.data_as(f64p), K.ctypes.data_as(f64p), None,
+                                   C.byref(st))
+        O.check(rc, "oracle_jk_range")
+        return time.perf_counter() - t, st.model_gflop
+
+    mid = nS // 2
+    t, g = run(mid, mid + 1)  # calibrate on one bra shell
+    width = 1
+    while t < budget_s / 8 and mid + width < nS:
+        width *= 2
+        t, g = run(mid, min(nS, mid + width))
+    rate = g / t if t > 0 else 0.0  # model GFLOP/s on this slice
+    builds = rate / total_gflop if total_gflop > 0 else 0.0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=builds, unit="builds/s", cores=threads, kind="port",
+                gflops_model=rate,
+                sample="bra shells %d..%d of %d (%.1f of %.1f model GFLOP) of the same FP64 build, %.1f s" %
+                       (mid, min(nS, mid + width) - 1, nS, g, total_gflop, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--carbons", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this engine has no multi-GPU path (the shell-quartet work "
+              "list would be sharded with one all-reduce of J/K per build); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    L = lib.load()
+    Z, xyz = geometry.alkane_chain(n_carbon=args.carbons, seed=1003)
+    basis = Basis("cc-pvdz-expanded", Z, xyz)
+    n = basis.n_aos
+    log("C%dH%d/cc-pVDZ: %d shells, %d functions" % (args.carbons, 2 * args.carbons + 2, basis.n_shells, n))
+    scf_kw = dict(energy_tol=1e-8, schwarz_thresh=1e-10, density_thresh=1e-10, diis_size=6, max_iter=60)
+
+    r64 = lib.scf_run(L, basis, options=L.default_options(**scf_kw))
+    log("FP64 SCF: E=%.10f it=%d conv=%d wall=%.2fs (J/K %.2fs, diag %.2fs)" % (
+        r64["e_total"], r64["iterations"], r64["converged"], r64["time_total_s"], r64["time_jk_s"], r64["time_diag_s"]))
+    Da = r64["D_alpha"]
+
+    eng = lib.Engine(L, basis)
+    Q = eng.schwarz()
+    cut, table = autotune.tune_cut(eng, Da, target=1e-7, Q=Q)
+    for row in table:
+        log("cut %.0e: |dE2| %.2e, %.3fs, fp32 %d / %d" % row)
+    if cut == 0.0:
+        cut = 1e-6
+    rmx = lib.scf_run(L, basis, options=L.default_options(precision=lib.HFX_MIXED, fp32_cut=cut, final_fp64=1,
+                                                          **scf_kw))
+    dE_mixed = abs(rmx["e_total"] - r64["e_total"])
+    dE_mixed_stop = abs(rmx["e_total_mixed"] - r64["e_total"])
+    dF = float(np.abs(rmx["F_alpha"] - r64["F_alpha"]).max())
+    log("mixed SCF (cut %.0e): E=%.10f |dE|=%.2e (at the mixed stop %.2e) max|dF|=%.2e it=%d+%d wall=%.2fs" % (
+        cut, rmx["e_total"], dE_mixed, dE_mixed_stop, dF, rmx["iterations_mixed"],
+        rmx["iterations"] - rmx["iterations_mixed"], rmx["time_total_s"]))
+
+    bufs = (device.DeviceArray.from_host(2 * Da), device.DeviceArray.from_host(Da), device.DeviceArray(n * n),
+            device.DeviceArray(n * n))
+    t64, st64, c64, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q)
+    J64, K64 = bufs[2].download(), bufs[3].download()
+    tmx, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q)
+    Jmx, Kmx = bufs[2].download(), bufs[3].download()
+    dF_build = float(max(np.abs(Jmx - J64).max(), np.abs(Kmx - K64).max()))
+    dE_build = abs(float(np.sum(Da.reshape(-1, order="F") * ((Jmx - Kmx) - (J64 - K64)))))
+    b64 = args.steps / t64
+    bmx = args.steps / tmx
+    log("FP64 builds/s %.3f (%.2f ms, device %.2f ms, %.1f model GFLOP, %d quartets)" % (
+        b64, 1e3 * t64 / args.steps, st64["device_ms"], st64["model_gflop"], st64["n_quartets"]))
+    log("mixed builds/s %.3f (%.2f ms, device %.2f ms, fp32 quartets %d)" % (
+        bmx, 1e3 * tmx / args.steps, stmx["device_ms"], stmx["n_fp32"]))
+
+    # dominant kernel group of the headline (mixed) build
+    groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
+    ms, prec, k = max(groups)
+    gfl = (stmx["class_gflop64"] if prec == "f64" else stmx["class_gflop32"])[k]
+    achieved = gfl / ms if ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
+    roof = dict(bound="mfma", pipe="VALU %s (MI355X peak listed for %s)" % (prec, prec), kernel="k_eri %s %s" % (
+        CLASS_NAMES[k], prec), achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
+        frac=achieved / PEAK_TFLOPS[prec], traffic=None, launch_ms=ms, launch_model_gflop=gfl,
+        build_achieved_tflops=stmx["model_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
+    per_class = {CLASS_NAMES[i]: [round(cm64[i], 3), round(cm32[i], 3)] for i in range(21) if cm64[i] + cm32[i] > 0}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(basis, Da, Q, st64["model_gflop"], args.cpu_budget)
+        log("cpu baseline: %s" % cpu)
+
+    out = {
+        "metric": "Fock builds/s (C20H42/cc-pVDZ, mixed FP32/FP64, |dE| <= 1e-6 Eh after the FP64 restart)",
+        "value": bmx,
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tmx / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "C%dH%d/cc-pVDZ expanded (%d Cartesian functions), full direct Fock build at the "
+                               "converged FP64 density, Schwarz 1e-10, density 1e-10" % (
+                                   args.carbons, 2 * args.carbons + 2, n),
+                   "fp32_cut": cut, "quartets": st64["n_quartets"], "model_gflop": st64["model_gflop"]},
+        "fp64_builds_per_s": b64,
+        "speedup_mixed_vs_fp64": bmx / b64,
+        "scf_wall_s_fp64": r64["time_total_s"],
+        "scf_wall_s_mixed": rmx["time_total_s"],
+        "scf_iterations_fp64": r64["iterations"],
+        "scf_iterations_mixed": [rmx["iterations_mixed"], rmx["iterations"] - rmx["iterations_mixed"]],
+        "e_fp64": r64["e_total"],
+        "dE_mixed_scf": dE_mixed,
+        "dE_mixed_at_fp32_stop": dE_mixed_stop,
+        "max_dF_mixed_scf": dF,
+        "build_dE_mixed": dE_build,
+        "build_max_dJK_mixed": dF_build,
+        "class_ms_f64_f32": per_class,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+"""Build the HIP engine library ``lib/libhfx_mi355x.so`` for gfx950.
+
+Translation units:
+  csrc/engine.hip             host engine, screening / one-electron / symmetrization kernels
+  csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
+                              are fully unrolled, so they are compiled in parallel slices
+  csrc/hfx_scf.cpp            host SCF driver (g++ with OpenMP)
+
+Usage: python build.py [-j N]
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libhfx_mi355x.so")
+INC = os.path.join(os.path.dirname(HERE), "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("HFX_ARCH", "gfx950")
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-I" + INC,
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+CXXFLAGS = ["-O3", "-std=c++20", "-fPIC", "-fopenmp", "-I" + INC, "-Wall"]
+
+ACC_MAX = 36          # must match k_eri.hip
+CHUNKS_PER_TU = 3     # chunk kernels per translation unit (x2 for FP64/FP32)
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def combo_classes():
+    """(combo index, la, lb, lc, ld, number of chunk kernels) for the 21 classes."""
+    out = []
+    for k in range(21):
+        p1 = 0
+        while (p1 + 1) * (p1 + 2) // 2 <= k:
+            p1 += 1
+        p2 = k - p1 * (p1 + 1) // 2
+
+        def pl(pc):
+            a = 0
+            while (a + 1) * (a + 2) // 2 <= pc:
+                a += 1
+            return a, pc - a * (a + 1) // 2
+
+        la, lb = pl(p1)
+        lc, ld = pl(p2)
+        nab, ncd = ncart(la) * ncart(lb), ncart(lc) * ncart(ld)
+        cbmax = max(1, ACC_MAX // ncd)
+        nch = (nab + cbmax - 1) // cbmax
+        out.append((k, la, lb, lc, ld, nch))
+    return out
+
+
+def jobs():
+    srcs = []
+    srcs.append(("engine", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "engine.hip")]))
+    for (k, la, lb, lc, ld, nch) in combo_classes():
+        for b in range(0, nch, CHUNKS_PER_TU):
+            e = min(nch, b + CHUNKS_PER_TU)
+            name = "k_eri_%02d_%02d" % (k, b)
+            srcs.append((name, [HIPCC] + HIPFLAGS + ["-DHFX_COMBO=%d" % k, "-DHFX_CHUNK_BEGIN=%d" % b,
+                                                     "-DHFX_CHUNK_END=%d" % e, "-c",
+                                                     os.path.join(CSRC, "k_eri.hip")]))
+    srcs.append(("hfx_scf", ["g++"] + CXXFLAGS + ["-c", os.path.join(CSRC, "hfx_scf.cpp")]))
+    return srcs
+
+
+def _run(name, cmd):
+    obj = os.path.join(BUILD, name + ".o")
+    t = time.time()
+    r = subprocess.run(cmd + ["-o", obj], capture_output=True, text=True)
+    msg = "\n".join(x for x in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in x)
+    return name, obj, r.returncode, msg, time.time() - t
+
+
+def build(jobs_n=None, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    todo = jobs()
+    n = jobs_n or min(16, os.cpu_count() or 8)
+    objs, failed = [], []
+    t0 = time.time()
+    # the biggest classes first, so the long compiles start early
+    order = sorted(todo, key=lambda j: (not j[0].startswith("k_eri_2"), not j[0].startswith("k_eri_1"), j[0]))
+    with cf.ThreadPoolExecutor(max_workers=n) as ex:
+        futs = [ex.submit(_run, name, cmd) for name, cmd in order]
+        for f in cf.as_completed(futs):
+            name, obj, rc, msg, dt = f.result()
+            if rc != 0:
+                failed.append((name, msg))
+            else:
+                objs.append(obj)
+                if verbose:
+                    print("  %-16s %6.1fs" % (name, dt), flush=True)
+    if failed:
+        for name, msg in failed:
+            print("---- %s ----\n%s" % (name, msg[-4000:]), file=sys.stderr)
+        raise RuntimeError("HIP build failed: " + ", ".join(n for n, _ in failed))
+    objs.sort()
+    link = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + \
+        ["-lgomp", "-Wl,-Bsymbolic", "-Wl,--no-undefined"]
+    r = subprocess.run(link, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stderr[-4000:])
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print("built %s in %.1fs (%d objects)" % (LIB, time.time() - t0, len(objs)), flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args()
+    build(a.j, verbose=True)

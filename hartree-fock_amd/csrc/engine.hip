@@ -1,0 +1,1353 @@
+// engine.hip - the MI355X direct-SCF Fock-build engine behind include/hfx.h.
+//
+// Host side: the flat basis (IntegralEngine::IntegralEngine, src/IntegralEngine.cpp:13-75),
+// shell pairs binned by angular-momentum class, primitive-pair data (computePrimitivePairData,
+// src/IntegralEngine.cpp:995-1022, precomputed once per geometry), the Boys table, and the
+// per-build pipeline.  Device side (this file): shell-pair density maxima, the screening
+// kernel that writes the per-class work lists, the symmetrization of J/K and the
+// one-electron integrals.  The ERI + J/K kernels live in k_eri.hip.
+//
+// One J/K build (src/SCF.cpp:523-744) on the engine's stream:
+//   k_dmax        shell-pair maxima of |dD| (SCF.cpp:543-575)
+//   k_screen      per class: canonical quartets passing Q(a,b)Q(c,d) >= tau_S and the
+//                 density test (SCF.cpp:602-623), appended to the class's work list
+//                 (FP64 part from the region start, FP32 part from the region end)
+//   k_ranges      list ranges from the append counters
+//   k_eri<...>    contracted ERIs + digestion into AJ/AK (k_eri.hip)
+//   k_symm        J = AJ + AJ^T, K = AK + AK^T (the reference's selfadjointView, SCF.cpp:727-728)
+// No host synchronization inside the pipeline: kernels read list sizes from device memory.
+#include "../../include/hfx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "md.hpp"
+
+using namespace hfx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t err_ = (x);                                                             \
+        if (err_ != hipSuccess)                                                            \
+            return fail(HFX_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(err_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------
+// Registry of ERI class kernels (filled by the k_eri translation units at load time).
+// ---------------------------------------------------------------------------------------
+struct PartTable {
+    EriPartFn fn[N_COMBOS][64];
+    int begin[N_COMBOS][64];
+    int end[N_COMBOS][64];
+    int nparts[N_COMBOS];
+    int nch[N_COMBOS];
+};
+PartTable& parts() {
+    static PartTable t;  // zero-initialized before any dynamic initializer runs
+    return t;
+}
+
+}  // namespace
+
+namespace hfx {
+
+bool register_eri_part(int combo, int begin, int end, int nch, EriPartFn fn) {
+    PartTable& t = parts();
+    if (combo < 0 || combo >= N_COMBOS || t.nparts[combo] >= 64) return false;
+    const int k = t.nparts[combo]++;
+    t.fn[combo][k] = fn;
+    t.begin[combo][k] = begin;
+    t.end[combo][k] = end;
+    t.nch[combo] = nch;
+    return true;
+}
+
+int eri_chunks(int combo) {
+    PartTable& t = parts();
+    const int n = t.nch[combo];
+    std::vector<int> cover(n > 0 ? n : 0, 0);
+    for (int k = 0; k < t.nparts[combo]; ++k)
+        for (int c = t.begin[combo][k]; c < t.end[combo][k] && c < n; ++c) cover[c]++;
+    for (int c = 0; c < n; ++c)
+        if (cover[c] != 1) return -1;
+    return n > 0 ? n : -1;
+}
+
+void launch_eri(int combo, int fp32, const EriArgs& a, int grid, hipStream_t s) {
+    PartTable& t = parts();
+    for (int k = 0; k < t.nparts[combo]; ++k) t.fn[combo][k](fp32, a, grid, s);
+}
+
+}  // namespace hfx
+
+namespace {
+
+constexpr int CMAX = 6;  // Cartesian components of a d shell
+
+// ---------------------------------------------------------------------------------------
+// Boys table: F_m(i * 0.1), i = 0..300, m = 0..16, from the series
+// F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double.
+// ---------------------------------------------------------------------------------------
+long double boys_series_ld(int m, long double T) {
+    long double term = 1.0L / (2 * m + 1), sum = term;
+    for (int k = 1; k < 400; ++k) {
+        term *= 2.0L * T / (2 * m + 2 * k + 1);
+        sum += term;
+        if (term < 1e-22L * sum) break;
+    }
+    return sum * expl(-T);
+}
+
+// ---------------------------------------------------------------------------------------
+// Device kernels of the build pipeline.
+// ---------------------------------------------------------------------------------------
+__global__ void k_fill(double* p, long long n, double v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+struct DmaxArgs {
+    const int* pair_a;
+    const int* pair_b;
+    const int* sh_ao;
+    const int* sh_nao;
+    const long long* sh_mbase;
+    const int* sh_mn;
+    const int* sh_local;
+    const long long* sh_qbase;
+    const int* sh_ns;
+    const double *Dt, *Da, *Db;
+    double *Mt, *Ma, *Mb;
+    int n_pairs;
+};
+
+// max |dD| over the AO block (rows of the higher-index shell, columns of the lower one, as
+// SCF.cpp:550-566), stored symmetrically in the molecule's nS x nS table.
+__global__ void k_dmax(DmaxArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n_pairs) return;
+    const int s1 = max(a.pair_a[p], a.pair_b[p]), s2 = min(a.pair_a[p], a.pair_b[p]);
+    const long long base = a.sh_mbase[s1];
+    const long long N = a.sh_mn[s1];
+    const int r0 = a.sh_ao[s1], nr = a.sh_nao[s1], c0 = a.sh_ao[s2], nc = a.sh_nao[s2];
+    double mt = 0.0, ma = 0.0, mb = 0.0;
+    for (int r = r0; r < r0 + nr; ++r)
+        for (int c = c0; c < c0 + nc; ++c) {
+            const long long x = base + r + c * N;
+            mt = fmax(mt, fabs(a.Dt[x]));
+            ma = fmax(ma, fabs(a.Da[x]));
+            if (a.Db) mb = fmax(mb, fabs(a.Db[x]));
+        }
+    const long long q = a.sh_qbase[s1];
+    const int ns = a.sh_ns[s1], l1 = a.sh_local[s1], l2 = a.sh_local[s2];
+    a.Mt[q + l1 + (long long)l2 * ns] = a.Mt[q + l2 + (long long)l1 * ns] = mt;
+    a.Ma[q + l1 + (long long)l2 * ns] = a.Ma[q + l2 + (long long)l1 * ns] = ma;
+    if (a.Db) a.Mb[q + l1 + (long long)l2 * ns] = a.Mb[q + l2 + (long long)l1 * ns] = mb;
+}
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+struct ScreenArgs {
+    // bra class list
+    const int* bra_ids;   // pair ids in sorted order
+    int n_bra;
+    int bra_pos0;         // position of bra_ids[0] in its class list (0)
+    // ket class list
+    const int* ket_ids;
+    const double* ket_q;  // Q of each ket-list position
+    const int* ket_seg_start;
+    const int* ket_seg_end;
+    const int* ket_molseg;  // [mol] -> first segment, [mol+1] -> end (per ket class)
+    bool diagonal;          // bra class == ket class: only ket positions <= bra position
+    // per pair / shell data
+    const int* pair_a;
+    const int* pair_b;
+    const int* pair_np;
+    const int* pair_mol;
+    const double* pair_q;
+    const int* sh_local;
+    const long long* sh_qbase;
+    const int* sh_ns;
+    const double *Mt, *Ma, *Mb;
+    double tauS, tauD, cut;
+    bool mixed;
+    long long nS_total;
+    // output
+    uint2* list;
+    long long reg_lo, reg_hi;
+    unsigned long long* cnt;  // [0] FP64 count, [1] FP32 count of this class
+    unsigned long long* stats;  // [0] hash sum, [1] hash xor, [2]/[3] primitive quartets FP64/FP32
+};
+
+// One wavefront per bra pair.  Ket candidates: for every nprim segment of the bra's
+// molecule in the ket class, the Schwarz prefix (segments are sorted by Q descending, so
+// Q(ab)Q(cd) >= tau_S holds on a prefix), then the density test per candidate.
+__global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int bpos = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (bpos >= a.n_bra) return;  // uniform per wavefront; no barriers below
+    const int pid = a.bra_ids[bpos];
+    const int sa = a.pair_a[pid], sb = a.pair_b[pid];
+    const int mol = a.pair_mol[pid];
+    const double qab = a.pair_q[pid];
+    const long long qb = a.sh_qbase[sa];
+    const long long ns = a.sh_ns[sa];
+    const int la = a.sh_local[sa], lb = a.sh_local[sb];
+    const double tab2 = 2.0 * a.Mt[qb + la + (long long)lb * ns];
+    const long long npa = a.pair_np[pid];
+    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0;
+    const int sg0 = a.ket_molseg[mol], sg1 = a.ket_molseg[mol + 1];
+    for (int sg = sg0; sg < sg1; ++sg) {
+        const int s0 = a.ket_seg_start[sg];
+        int s1 = a.ket_seg_end[sg];
+        if (a.diagonal) s1 = min(s1, a.bra_pos0 + bpos + 1);
+        if (s1 <= s0) continue;
+        // Schwarz prefix: first x in [s0, s1) with qab * Q(x) < tau_S.
+        int lo = s0, hi = s1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (qab * a.ket_q[mid] < a.tauS)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const int xend = lo;
+        for (int x0 = s0; x0 < xend; x0 += 64) {
+            const int x = x0 + lane;
+            bool keep = false, f32 = false;
+            int qid = 0;
+            if (x < xend) {
+                qid = a.ket_ids[x];
+                const int sc = a.pair_a[qid], sd = a.pair_b[qid];
+                const int lc = a.sh_local[sc], ld = a.sh_local[sd];
+                double mx = fmax(tab2, 2.0 * a.Mt[qb + lc + (long long)ld * ns]);
+                mx = fmax(mx, a.Ma[qb + la + (long long)lc * ns]);
+                mx = fmax(mx, a.Ma[qb + la + (long long)ld * ns]);
+                mx = fmax(mx, a.Ma[qb + lb + (long long)lc * ns]);
+                mx = fmax(mx, a.Ma[qb + lb + (long long)ld * ns]);
+                if (a.Mb) {
+                    mx = fmax(mx, a.Mb[qb + la + (long long)lc * ns]);
+                    mx = fmax(mx, a.Mb[qb + la + (long long)ld * ns]);
+                    mx = fmax(mx, a.Mb[qb + lb + (long long)lc * ns]);
+                    mx = fmax(mx, a.Mb[qb + lb + (long long)ld * ns]);
+                }
+                keep = !(mx < a.tauD);
+                if (keep) {
+                    f32 = a.mixed && (qab * a.ket_q[x] * mx < a.cut);
+                    // Canonical (reference loop order) form of the quartet for the hash.
+                    long long A = max(sa, sb), B = min(sa, sb), C = max(sc, sd), D = min(sc, sd);
+                    if (A * (A + 1) / 2 + B < C * (C + 1) / 2 + D) {
+                        long long t = A;
+                        A = C;
+                        C = t;
+                        t = B;
+                        B = D;
+                        D = t;
+                    }
+                    const unsigned long long h =
+                        splitmix64((unsigned long long)(((A * a.nS_total + B) * a.nS_total + C) * a.nS_total + D));
+                    h_sum += h;
+                    h_xor ^= h;
+                    if (f32)
+                        prims32 += (unsigned long long)(npa * a.pair_np[qid]);
+                    else
+                        prims += (unsigned long long)(npa * a.pair_np[qid]);
+                }
+            }
+            const unsigned long long m64 = __ballot(keep && !f32);
+            const unsigned long long m32 = __ballot(keep && f32);
+            unsigned long long b64 = 0, b32 = 0;
+            if (lane == 0) {
+                if (m64) b64 = atomicAdd(&a.cnt[0], (unsigned long long)__popcll(m64));
+                if (m32) b32 = atomicAdd(&a.cnt[1], (unsigned long long)__popcll(m32));
+            }
+            b64 = __shfl(b64, 0);
+            b32 = __shfl(b32, 0);
+            if (keep) {
+                const uint2 ent = make_uint2((unsigned)pid, (unsigned)qid);
+                if (!f32)
+                    a.list[a.reg_lo + (long long)(b64 + __popcll(m64 & lt_mask))] = ent;
+                else
+                    a.list[a.reg_hi - 1 - (long long)(b32 + __popcll(m32 & lt_mask))] = ent;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        h_sum += __shfl_xor(h_sum, off);
+        h_xor ^= __shfl_xor(h_xor, off);
+        prims += __shfl_xor(prims, off);
+        prims32 += __shfl_xor(prims32, off);
+    }
+    if (lane == 0) {
+        if (h_sum) atomicAdd(&a.stats[0], h_sum);
+        if (h_xor) atomicXor(&a.stats[1], h_xor);
+        if (prims) atomicAdd(&a.stats[2], prims);
+        if (prims32) atomicAdd(&a.stats[3], prims32);
+    }
+}
+
+// ranges[2k..2k+1] = FP64 entries of class k, ranges[42+2k..] = FP32 entries.
+__global__ void k_ranges(const unsigned long long* cnt, const long long* reg, long long* ranges) {
+    const int k = threadIdx.x;
+    if (k >= N_COMBOS) return;
+    ranges[2 * k] = reg[k];
+    ranges[2 * k + 1] = reg[k] + (long long)cnt[2 * k];
+    ranges[2 * N_COMBOS + 2 * k] = reg[k + 1] - (long long)cnt[2 * k + 1];
+    ranges[2 * N_COMBOS + 2 * k + 1] = reg[k + 1];
+}
+
+// out = A + A^T within every molecule block (blockIdx.y = molecule).
+__global__ void k_symm(const double* A, double* out, const long long* mbase, const int* mn) {
+    const int m = blockIdx.y;
+    const long long base = mbase[m];
+    const long long N = mn[m];
+    const long long n = N * N;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e % N, c = e / N;
+        out[base + e] = A[base + r + c * N] + A[base + c + r * N];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// One-electron integrals (IntegralEngine.cpp:78-368), one thread per shell pair of a class.
+// KIND 0: overlap, 1: kinetic, 2: nuclear attraction.
+// ---------------------------------------------------------------------------------------
+struct OneArgs {
+    const int* ids;  // pair ids of one class
+    int n;
+    const int* pair_a;
+    const int* pair_b;
+    const int* sh_po;
+    const int* sh_np;
+    const int* sh_no;
+    const int* sh_ao;
+    const long long* sh_mbase;
+    const int* sh_mn;
+    const int* sh_mol;
+    const double* ctr;
+    const double* alpha;
+    const double* coeff;
+    const double* norm;
+    const double* atom_z;
+    const double* atom_xyz;
+    const int* mol_atom0;  // [mol], [mol+1]: atom range of a molecule
+    const double* boys;    // full table, BOYS_ORDERS per grid point
+    double* out;
+};
+
+template <int LA, int LB, int KIND>
+__global__ __launch_bounds__(128) void k_one(OneArgs o) {
+    using namespace md;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= o.n) return;
+    const int pid = o.ids[t];
+    const int sa = o.pair_a[pid], sb = o.pair_b[pid];
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    constexpr int LBE = (KIND == 1) ? LB + 2 : LB;
+    constexpr int L = LA + LB;
+    const double Ax = o.ctr[3 * sa], Ay = o.ctr[3 * sa + 1], Az = o.ctr[3 * sa + 2];
+    const double Bx = o.ctr[3 * sb], By = o.ctr[3 * sb + 1], Bz = o.ctr[3 * sb + 2];
+    const double AB2 = (Ax - Bx) * (Ax - Bx) + (Ay - By) * (Ay - By) + (Az - Bz) * (Az - Bz);
+    double acc[NA][NB];
+    sfor<NA>([&](auto i) { sfor<NB>([&](auto j) { acc[i][j] = 0.0; }); });
+    const int npa = o.sh_np[sa], npb = o.sh_np[sb];
+    for (int pa = 0; pa < npa; ++pa) {
+        const double al = o.alpha[o.sh_po[sa] + pa];
+        for (int pb = 0; pb < npb; ++pb) {
+            const double be = o.alpha[o.sh_po[sb] + pb];
+            const double p = al + be, oop = 1.0 / p;
+            const double Px = (al * Ax + be * Bx) * oop, Py = (al * Ay + be * By) * oop, Pz = (al * Az + be * Bz) * oop;
+            const double K = exp(-al * be * oop * AB2);
+            const double cp = o.coeff[o.sh_po[sa] + pa] * o.coeff[o.sh_po[sb] + pb];
+            Ecoef<LA, LBE, double> Ex, Ey, Ez;
+            hermite_e<LA, LBE>(0.5 * oop, Px - Ax, Px - Bx, Ex);
+            hermite_e<LA, LBE>(0.5 * oop, Py - Ay, Py - By, Ey);
+            hermite_e<LA, LBE>(0.5 * oop, Pz - Az, Pz - Bz, Ez);
+            const double* na = o.norm + o.sh_no[sa] + pa * NA;
+            const double* nb = o.norm + o.sh_no[sb] + pb * NB;
+            if constexpr (KIND == 0 || KIND == 1) {
+                const double pref = pow(M_PI * oop, 1.5) * K * cp;
+                sfor<NA>([&](auto ia_) {
+                    constexpr int ia = decltype(ia_)::value;
+                    constexpr int ax = cart_i(LA, ia), ay = cart_j(LA, ia), az = cart_k(LA, ia);
+                    sfor<NB>([&](auto ib_) {
+                        constexpr int ib = decltype(ib_)::value;
+                        constexpr int bx = cart_i(LB, ib), by = cart_j(LB, ib), bz = cart_k(LB, ib);
+                        const double sx = Ex.e[ax][bx][0], sy = Ey.e[ay][by][0], sz = Ez.e[az][bz][0];
+                        double v;
+                        if constexpr (KIND == 0) {
+                            v = sx * sy * sz;
+                        } else {
+                            // Laplacian on b (IntegralEngine.cpp:217-233)
+                            const double t1 = be * (2 * (bx + by + bz) + 3) * sx * sy * sz;
+                            const double t2 = -2.0 * be * be *
+                                              (Ex.e[ax][bx + 2][0] * sy * sz + sx * Ey.e[ay][by + 2][0] * sz +
+                                               sx * sy * Ez.e[az][bz + 2][0]);
+                            double t3 = 0.0;
+                            if constexpr (bx >= 2) t3 -= bx * (bx - 1) * Ex.e[ax][bx - 2][0] * sy * sz;
+                            if constexpr (by >= 2) t3 -= by * (by - 1) * sx * Ey.e[ay][by - 2][0] * sz;
+                            if constexpr (bz >= 2) t3 -= bz * (bz - 1) * sx * sy * Ez.e[az][bz - 2][0];
+                            v = t1 + t2 + 0.5 * t3;
+                        }
+                        acc[ia][ib] += na[ia] * nb[ib] * pref * v;
+                    });
+                });
+            } else {
+                const double pref = (2.0 * M_PI * oop) * K * cp;
+                const int mol = o.sh_mol[sa];
+                for (int at = o.mol_atom0[mol]; at < o.mol_atom0[mol + 1]; ++at) {
+                    const double X = Px - o.atom_xyz[3 * at], Y = Py - o.atom_xyz[3 * at + 1],
+                                 Z = Pz - o.atom_xyz[3 * at + 2];
+                    const double T = p * (X * X + Y * Y + Z * Z);
+                    double F[L + 1];
+                    boys<L, double, BOYS_ORDERS>(T, o.boys + BoysSlice<L, double>::first, F);
+                    double base[L + 1];
+                    double pw = -o.atom_z[at] * pref;
+                    sfor<L + 1>([&](auto n) {
+                        base[n] = pw * F[n];
+                        pw *= -2.0 * p;
+                    });
+                    double Rt[nherm(L)];
+                    hermite_r<L>(base, X, Y, Z, Rt);
+                    sfor<NA>([&](auto ia_) {
+                        constexpr int ia = decltype(ia_)::value;
+                        constexpr int ax = cart_i(LA, ia), ay = cart_j(LA, ia), az = cart_k(LA, ia);
+                        sfor<NB>([&](auto ib_) {
+                            constexpr int ib = decltype(ib_)::value;
+                            constexpr int bx = cart_i(LB, ib), by = cart_j(LB, ib), bz = cart_k(LB, ib);
+                            const double v = sum_of<az + bz + 1>([&](auto v_) {
+                                constexpr int w = decltype(v_)::value;
+                                return Ez.e[az][bz][w] * sum_of<ay + by + 1>([&](auto u_) {
+                                           constexpr int u = decltype(u_)::value;
+                                           return Ey.e[ay][by][u] * sum_of<ax + bx + 1>([&](auto t_) {
+                                                      constexpr int tt = decltype(t_)::value;
+                                                      return Ex.e[ax][bx][tt] * Rt[hidx(tt, u, w)];
+                                                  });
+                                       });
+                            });
+                            acc[ia][ib] += na[ia] * nb[ib] * v;
+                        });
+                    });
+                }
+            }
+        }
+    }
+    const long long base = o.sh_mbase[sa];
+    const long long N = o.sh_mn[sa];
+    const int oA = o.sh_ao[sa], oB = o.sh_ao[sb];
+    sfor<NA>([&](auto ia) {
+        sfor<NB>([&](auto ib) {
+            o.out[base + (oA + ia) + (oB + ib) * N] = acc[ia][ib];
+            o.out[base + (oB + ib) + (oA + ia) * N] = acc[ia][ib];
+        });
+    });
+}
+
+template <int KIND>
+void launch_one(int pc, const OneArgs& o, hipStream_t s) {
+    const int grid = (o.n + 127) / 128;
+    if (o.n == 0) return;
+    switch (pc) {
+        case 0: hipLaunchKernelGGL((k_one<0, 0, KIND>), dim3(grid), dim3(128), 0, s, o); break;
+        case 1: hipLaunchKernelGGL((k_one<1, 0, KIND>), dim3(grid), dim3(128), 0, s, o); break;
+        case 2: hipLaunchKernelGGL((k_one<1, 1, KIND>), dim3(grid), dim3(128), 0, s, o); break;
+        case 3: hipLaunchKernelGGL((k_one<2, 0, KIND>), dim3(grid), dim3(128), 0, s, o); break;
+        case 4: hipLaunchKernelGGL((k_one<2, 1, KIND>), dim3(grid), dim3(128), 0, s, o); break;
+        case 5: hipLaunchKernelGGL((k_one<2, 2, KIND>), dim3(grid), dim3(128), 0, s, o); break;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Device memory owned by an engine.
+// ---------------------------------------------------------------------------------------
+struct DevArena {
+    std::vector<void*> ptrs;
+    template <class T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    ~DevArena() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+struct hfx_engine {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevArena mem;
+
+    // molecules
+    int n_mol = 0;
+    std::vector<int> mol_N, mol_nS, mol_shell0, mol_atom0;
+    std::vector<long long> mol_mbase, mol_qbase;
+    long long n_elems = 0, n_qelems = 0;
+    int nS = 0, N_tot = 0;
+
+    // shells (global index), host
+    std::vector<int> sh_l, sh_np, sh_po, sh_no, sh_ao, sh_nao, sh_mol, sh_local, sh_ns, sh_mn;
+    std::vector<long long> sh_mbase, sh_qbase;
+    std::vector<double> sh_ctr, sh_cf, alpha, coeff, norm;
+
+    // pairs
+    int n_pairs = 0;
+    std::vector<int> pair_a, pair_b, pair_ps, pair_np, pair_pc, pair_mol;
+    std::map<long long, int> pair_of;
+    long long n_pp = 0;
+
+    // class lists (sorted) and segments
+    std::vector<int> cls_ids[N_PAIR_CLASSES];
+    std::vector<double> cls_q[N_PAIR_CLASSES];
+    std::vector<int> seg_start[N_PAIR_CLASSES], seg_end[N_PAIR_CLASSES], molseg[N_PAIR_CLASSES];
+    std::vector<double> pair_q;
+    std::vector<double> sort_q;  // the Q table the current order was built from
+    std::vector<double> own_q;   // the engine's Schwarz table
+    std::vector<long long> reg;  // class regions in the work list, [N_COMBOS + 1]
+
+    // device copies
+    int *d_pair_a = nullptr, *d_pair_b = nullptr, *d_pair_ps = nullptr, *d_pair_np = nullptr, *d_pair_mol = nullptr;
+    double* d_pair_q = nullptr;
+    PrimPair64* d_pp64 = nullptr;
+    PrimPair32* d_pp32 = nullptr;
+    int *d_sh_ao = nullptr, *d_sh_nao = nullptr, *d_sh_mn = nullptr, *d_sh_local = nullptr, *d_sh_ns = nullptr;
+    int *d_sh_po = nullptr, *d_sh_np = nullptr, *d_sh_no = nullptr, *d_sh_mol = nullptr;
+    long long *d_sh_mbase = nullptr, *d_sh_qbase = nullptr;
+    double *d_sh_cf = nullptr, *d_ctr = nullptr, *d_alpha = nullptr, *d_coeff = nullptr, *d_norm = nullptr;
+    int* d_cls_ids[N_PAIR_CLASSES] = {};
+    double* d_cls_q[N_PAIR_CLASSES] = {};
+    int *d_seg_start[N_PAIR_CLASSES] = {}, *d_seg_end[N_PAIR_CLASSES] = {}, *d_molseg[N_PAIR_CLASSES] = {};
+    double* d_boys64 = nullptr;
+    float* d_boys32 = nullptr;
+    long long* d_mbase = nullptr;
+    int* d_mn = nullptr;
+    long long* d_reg = nullptr;
+
+    // per-build buffers
+    double *d_Dt = nullptr, *d_Da = nullptr, *d_Db = nullptr;
+    double *d_Mt = nullptr, *d_Ma = nullptr, *d_Mb = nullptr;
+    double *d_AJ = nullptr, *d_AKa = nullptr, *d_AKb = nullptr;
+    double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
+    uint2* d_list = nullptr;
+    long long list_cap = 0;
+    unsigned long long* d_cnt = nullptr;    // [2*N_COMBOS]
+    unsigned long long* d_stats = nullptr;  // [4 * N_COMBOS]
+    hipEvent_t cev[2 * N_COMBOS + 1] = {};
+    long long* d_ranges = nullptr;          // [4*N_COMBOS]
+
+    ~hfx_engine() {
+        for (hipEvent_t& x : cev)
+            if (x) (void)hipEventDestroy(x);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <class T>
+int upload(hfx_engine* e, T*& dst, const std::vector<T>& src) {
+    dst = e->mem.alloc<T>(src.size());
+    if (!dst) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+    if (!src.empty()) HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return HFX_OK;
+}
+
+int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+double model_flop_h(int la, int lb, int lc, int ld) {  // per primitive quartet, bra = larger pair class
+    auto nherm = [](int L) { return (L + 1) * (L + 2) * (L + 3) / 6; };
+    auto hsum = [](int l1, int l2) {
+        double s = 0;
+        for (int i1 = l1; i1 >= 0; --i1)
+            for (int j1 = l1 - i1; j1 >= 0; --j1)
+                for (int i2 = l2; i2 >= 0; --i2)
+                    for (int j2 = l2 - i2; j2 >= 0; --j2)
+                        s += (i1 + i2 + 1) * (j1 + j2 + 1) * ((l1 - i1 - j1) + (l2 - i2 - j2) + 1);
+        return s;
+    };
+    const int L = la + lb + lc + ld;
+    const double c4 = (double)(L + 4) * (L + 3) * (L + 2) * (L + 1) / 24.0;
+    return (20 + 3 * (L + 1)) + 3 * c4 + 2.0 * nherm(la + lb) * hsum(lc, ld) +
+           2.0 * ncart_h(lc) * ncart_h(ld) * hsum(la, lb) + 2.0 * ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+}
+
+// Sort every class list by (molecule, nprim desc, Q desc, id) using the Q table qt
+// (packed per molecule), build the nprim segments, and upload.
+int sort_pairs(hfx_engine* e, const std::vector<double>& qt) {
+    e->pair_q.resize(e->n_pairs);
+    for (int p = 0; p < e->n_pairs; ++p) {
+        const int a = e->pair_a[p], b = e->pair_b[p];
+        e->pair_q[p] = qt[e->sh_qbase[a] + e->sh_local[a] + (long long)e->sh_local[b] * e->sh_ns[a]];
+    }
+    for (int pc = 0; pc < N_PAIR_CLASSES; ++pc) {
+        std::vector<int>& ids = e->cls_ids[pc];
+        std::sort(ids.begin(), ids.end(), [&](int x, int y) {
+            if (e->pair_mol[x] != e->pair_mol[y]) return e->pair_mol[x] < e->pair_mol[y];
+            if (e->pair_np[x] != e->pair_np[y]) return e->pair_np[x] > e->pair_np[y];
+            if (e->pair_q[x] != e->pair_q[y]) return e->pair_q[x] > e->pair_q[y];
+            return x < y;
+        });
+        e->cls_q[pc].resize(ids.size());
+        for (size_t i = 0; i < ids.size(); ++i) e->cls_q[pc][i] = e->pair_q[ids[i]];
+        e->seg_start[pc].clear();
+        e->seg_end[pc].clear();
+        e->molseg[pc].assign(e->n_mol + 1, 0);
+        size_t i = 0;
+        for (int m = 0; m < e->n_mol; ++m) {
+            e->molseg[pc][m] = (int)e->seg_start[pc].size();
+            while (i < ids.size() && e->pair_mol[ids[i]] == m) {
+                size_t j = i;
+                while (j < ids.size() && e->pair_mol[ids[j]] == m && e->pair_np[ids[j]] == e->pair_np[ids[i]]) ++j;
+                e->seg_start[pc].push_back((int)i);
+                e->seg_end[pc].push_back((int)j);
+                i = j;
+            }
+        }
+        e->molseg[pc][e->n_mol] = (int)e->seg_start[pc].size();
+        auto put = [&](auto*& dst, const auto& src) -> int {
+            using T = typename std::decay_t<decltype(src)>::value_type;
+            if (!dst) {
+                dst = e->mem.alloc<T>(std::max<size_t>(src.size(), 1) + 1);
+                if (!dst) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+            }
+            if (!src.empty()) HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+            return HFX_OK;
+        };
+        int rc;
+        if ((rc = put(e->d_cls_ids[pc], ids)) || (rc = put(e->d_cls_q[pc], e->cls_q[pc]))) return rc;
+        // segment arrays can change length between sorts only through nprim runs, which are fixed
+        // by the pair set; their capacity is the number of pairs in the class.
+        if (!e->d_seg_start[pc]) {
+            e->d_seg_start[pc] = e->mem.alloc<int>(ids.size() + 1);
+            e->d_seg_end[pc] = e->mem.alloc<int>(ids.size() + 1);
+            if (!e->d_seg_start[pc] || !e->d_seg_end[pc]) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+        }
+        if (!e->seg_start[pc].empty()) {
+            HIPCHK(hipMemcpy(e->d_seg_start[pc], e->seg_start[pc].data(), e->seg_start[pc].size() * sizeof(int),
+                             hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e->d_seg_end[pc], e->seg_end[pc].data(), e->seg_end[pc].size() * sizeof(int),
+                             hipMemcpyHostToDevice));
+        }
+        if ((rc = put(e->d_molseg[pc], e->molseg[pc]))) return rc;
+    }
+    if (!e->d_pair_q) {
+        e->d_pair_q = e->mem.alloc<double>(e->n_pairs);
+        if (!e->d_pair_q) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+    }
+    HIPCHK(hipMemcpy(e->d_pair_q, e->pair_q.data(), e->n_pairs * sizeof(double), hipMemcpyHostToDevice));
+    e->sort_q = qt;
+    return HFX_OK;
+}
+
+EriArgs eri_args(hfx_engine* e) {
+    EriArgs a{};
+    a.pair_a = e->d_pair_a;
+    a.pair_b = e->d_pair_b;
+    a.pair_ps = e->d_pair_ps;
+    a.pair_np = e->d_pair_np;
+    a.pp64 = e->d_pp64;
+    a.pp32 = e->d_pp32;
+    a.sh_ao = e->d_sh_ao;
+    a.sh_mbase = e->d_sh_mbase;
+    a.sh_mn = e->d_sh_mn;
+    a.sh_cf = e->d_sh_cf;
+    a.boys64 = e->d_boys64;
+    a.boys32 = e->d_boys32;
+    return a;
+}
+
+int eri_grid(hfx_engine* e) { return e->n_cu * 8; }
+
+// Store-mode ERI blocks for a list of (bra pair, ket pair) entries of one class.
+int store_blocks(hfx_engine* e, int combo, const std::vector<uint2>& ents, std::vector<double>& out,
+                 std::vector<long long>& offs) {
+    const int pc1 = combo_pc1(combo), pc2 = combo_pc2(combo);
+    const long long blk = (long long)ncart_h(pc_la(pc1)) * ncart_h(pc_lb(pc1)) * ncart_h(pc_la(pc2)) * ncart_h(pc_lb(pc2));
+    const long long n = (long long)ents.size();
+    offs.resize(n);
+    for (long long i = 0; i < n; ++i) offs[i] = i * blk;
+    out.assign(n * blk, 0.0);
+    if (n == 0) return HFX_OK;
+    if (eri_chunks(combo) < 0) return fail(HFX_ERR_UNSUPPORTED, "ERI kernels missing for class " + std::to_string(combo));
+    DevArena tmp;
+    uint2* dl = tmp.alloc<uint2>(n);
+    long long* doff = tmp.alloc<long long>(n);
+    double* dout = tmp.alloc<double>(n * blk);
+    long long* drange = tmp.alloc<long long>(2);
+    if (!dl || !doff || !dout || !drange) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+    const long long rng[2] = {0, n};
+    HIPCHK(hipMemcpy(dl, ents.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(doff, offs.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(drange, rng, sizeof(rng), hipMemcpyHostToDevice));
+    EriArgs a = eri_args(e);
+    a.list = dl;
+    a.range = drange;
+    a.out = dout;
+    a.out_off = doff;
+    launch_eri(combo, 0, a, eri_grid(e), e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(out.data(), dout, n * blk * sizeof(double), hipMemcpyDeviceToHost));
+    return HFX_OK;
+}
+
+// Schwarz table from the diagonal quartets (ab|ab) of every pair (IntegralEngine.cpp:695-728).
+int compute_schwarz(hfx_engine* e) {
+    e->own_q.assign(e->n_qelems, 0.0);
+    for (int pc = 0; pc < N_PAIR_CLASSES; ++pc) {
+        const std::vector<int>& ids = e->cls_ids[pc];
+        if (ids.empty()) continue;
+        std::vector<uint2> ents(ids.size());
+        for (size_t i = 0; i < ids.size(); ++i) ents[i] = make_uint2(ids[i], ids[i]);
+        std::vector<double> out;
+        std::vector<long long> offs;
+        const int combo = combo_of(pc, pc);
+        int rc = store_blocks(e, combo, ents, out, offs);
+        if (rc) return rc;
+        const int nab = ncart_h(pc_la(pc)) * ncart_h(pc_lb(pc));
+        for (size_t i = 0; i < ids.size(); ++i) {
+            double mx = 0.0;
+            for (int ab = 0; ab < nab; ++ab) mx = std::max(mx, out[offs[i] + (long long)ab * nab + ab]);
+            const int a = e->pair_a[ids[i]], b = e->pair_b[ids[i]];
+            const long long q = e->sh_qbase[a];
+            const int ns = e->sh_ns[a];
+            const double v = std::sqrt(mx);
+            e->own_q[q + e->sh_local[a] + (long long)e->sh_local[b] * ns] = v;
+            e->own_q[q + e->sh_local[b] + (long long)e->sh_local[a] * ns] = v;
+        }
+    }
+    return HFX_OK;
+}
+
+bool finite_all(const double* x, long long n) {
+    for (long long i = 0; i < n; ++i)
+        if (!std::isfinite(x[i])) return false;
+    return true;
+}
+
+int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out) {
+    if (!out || !bases || n_mol <= 0) return fail(HFX_ERR_INVALID, "hfx_create: bad arguments");
+    *out = nullptr;
+    for (int c = 0; c < N_COMBOS; ++c)
+        if (eri_chunks(c) < 0) return fail(HFX_ERR_UNSUPPORTED, "library built without the kernels of class " + std::to_string(c));
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(HFX_ERR_INVALID, "hfx_create: no such device");
+    HIPCHK(hipSetDevice(device));
+    std::unique_ptr<hfx_engine> e(new hfx_engine());
+    e->device = device;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    e->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&e->ev0));
+    HIPCHK(hipEventCreate(&e->ev1));
+    for (hipEvent_t& x : e->cev) HIPCHK(hipEventCreate(&x));
+
+    // ---- flatten the molecules' bases ----
+    e->n_mol = n_mol;
+    int prim_base = 0, norm_base = 0;
+    for (int m = 0; m < n_mol; ++m) {
+        const hfx_basis* b = &bases[m];
+        if (b->n_shells <= 0 || !b->l || !b->n_prim || !b->prim_offset || !b->ao_offset || !b->norm_offset ||
+            !b->centers || !b->alpha || !b->coeff || !b->norm || !b->cart)
+            return fail(HFX_ERR_INVALID, "hfx_basis: missing field");
+        e->mol_shell0.push_back(e->nS);
+        e->mol_N.push_back(b->n_aos);
+        e->mol_nS.push_back(b->n_shells);
+        e->mol_mbase.push_back(e->n_elems);
+        e->mol_qbase.push_back(e->n_qelems);
+        int ao = 0;
+        for (int s = 0; s < b->n_shells; ++s) {
+            const int l = b->l[s];
+            if (l < 0 || l > HFX_MAX_L) return fail(HFX_ERR_UNSUPPORTED, "angular momentum above d");
+            const int nao = ncart_h(l);
+            if (b->ao_offset[s] != ao) return fail(HFX_ERR_INVALID, "hfx_basis: ao_offset not contiguous");
+            if (b->n_prim[s] <= 0) return fail(HFX_ERR_INVALID, "hfx_basis: shell without primitives");
+            for (int c = 0; c < nao; ++c) {  // the kernels assume the reference's Cartesian order
+                const int* cc = &b->cart[3 * (ao + c)];
+                if (cc[0] != md::cart_i(l, c) || cc[1] != md::cart_j(l, c) || cc[2] != md::cart_k(l, c))
+                    return fail(HFX_ERR_UNSUPPORTED, "hfx_basis: unexpected Cartesian order");
+            }
+            e->sh_l.push_back(l);
+            e->sh_np.push_back(b->n_prim[s]);
+            e->sh_po.push_back(prim_base + b->prim_offset[s]);
+            e->sh_no.push_back(norm_base + b->norm_offset[s]);
+            e->sh_ao.push_back(ao);
+            e->sh_nao.push_back(nao);
+            e->sh_mol.push_back(m);
+            e->sh_local.push_back(s);
+            e->sh_ns.push_back(b->n_shells);
+            e->sh_mn.push_back(b->n_aos);
+            e->sh_mbase.push_back(e->n_elems);
+            e->sh_qbase.push_back(e->n_qelems);
+            for (int x = 0; x < 3; ++x) e->sh_ctr.push_back(b->centers[3 * s + x]);
+            // normalization ratio of each component (independent of the primitive)
+            const double* nrm = &b->norm[b->norm_offset[s]];
+            for (int c = 0; c < CMAX; ++c) e->sh_cf.push_back(c < nao ? nrm[c] / nrm[0] : 0.0);
+            ao += nao;
+        }
+        if (ao != b->n_aos) return fail(HFX_ERR_INVALID, "hfx_basis: n_aos mismatch");
+        for (int p = 0; p < b->n_prims; ++p) {
+            e->alpha.push_back(b->alpha[p]);
+            e->coeff.push_back(b->coeff[p]);
+        }
+        for (int x = 0; x < b->n_norm; ++x) e->norm.push_back(b->norm[x]);
+        prim_base += b->n_prims;
+        norm_base += b->n_norm;
+        e->nS += b->n_shells;
+        e->N_tot += b->n_aos;
+        e->n_elems += (long long)b->n_aos * b->n_aos;
+        e->n_qelems += (long long)b->n_shells * b->n_shells;
+    }
+
+    // ---- shell pairs and primitive-pair data ----
+    std::vector<PrimPair64> pp64;
+    std::vector<int> cls_count(N_PAIR_CLASSES, 0);
+    for (int m = 0; m < n_mol; ++m) {
+        const int s0 = e->mol_shell0[m], s1 = s0 + e->mol_nS[m];
+        for (int i = s0; i < s1; ++i)
+            for (int j = s0; j <= i; ++j) {
+                int a = i, b = j;
+                if (e->sh_l[b] > e->sh_l[a]) std::swap(a, b);
+                const int pc = pair_class(e->sh_l[a], e->sh_l[b]);
+                const int pid = e->n_pairs++;
+                e->pair_a.push_back(a);
+                e->pair_b.push_back(b);
+                e->pair_pc.push_back(pc);
+                e->pair_mol.push_back(m);
+                e->pair_ps.push_back((int)pp64.size());
+                e->pair_np.push_back(e->sh_np[a] * e->sh_np[b]);
+                e->pair_of[(long long)j * e->nS + i] = pid;
+                e->cls_ids[pc].push_back(pid);
+                cls_count[pc]++;
+                const double* A = &e->sh_ctr[3 * a];
+                const double* B = &e->sh_ctr[3 * b];
+                const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+                for (int pa = 0; pa < e->sh_np[a]; ++pa)
+                    for (int pb = 0; pb < e->sh_np[b]; ++pb) {
+                        const double al = e->alpha[e->sh_po[a] + pa], be = e->alpha[e->sh_po[b] + pb];
+                        const double p = al + be, oop = 1.0 / p;
+                        PrimPair64 d;
+                        d.p = p;
+                        d.px = (al * A[0] + be * B[0]) * oop;
+                        d.py = (al * A[1] + be * B[1]) * oop;
+                        d.pz = (al * A[2] + be * B[2]) * oop;
+                        d.pax = d.px - A[0];
+                        d.pay = d.py - A[1];
+                        d.paz = d.pz - A[2];
+                        d.pbx = d.px - B[0];
+                        d.pby = d.py - B[1];
+                        d.pbz = d.pz - B[2];
+                        d.oo2p = 0.5 * oop;
+                        const double K = std::exp(-al * be * oop * AB2);
+                        const double ga = e->norm[e->sh_no[a] + pa * e->sh_nao[a]];
+                        const double gb = e->norm[e->sh_no[b] + pb * e->sh_nao[b]];
+                        d.kp = K * (e->coeff[e->sh_po[a] + pa] * ga) * (e->coeff[e->sh_po[b] + pb] * gb) * oop;
+                        pp64.push_back(d);
+                    }
+            }
+    }
+    e->n_pp = (long long)pp64.size();
+    std::vector<PrimPair32> pp32(pp64.size());
+    for (size_t i = 0; i < pp64.size(); ++i) {
+        const PrimPair64& s = pp64[i];
+        pp32[i] = PrimPair32{(float)s.p, (float)s.px, (float)s.py, (float)s.pz, (float)s.kp, (float)s.pax,
+                             (float)s.pay, (float)s.paz, (float)s.pbx, (float)s.pby, (float)s.pbz, (float)s.oo2p};
+    }
+
+    // ---- Boys tables ----
+    std::vector<double> b64((size_t)md::BOYS_POINTS * md::BOYS_ORDERS);
+    std::vector<float> b32((size_t)md::BOYS_POINTS * md::BOYS_ORDERS);
+    for (int i = 0; i < md::BOYS_POINTS; ++i) {
+        const double T64 = i * 0.1;
+        const float T32 = (float)i * 0.1f;
+        for (int m = 0; m < md::BOYS_ORDERS; ++m) {
+            b64[(size_t)i * md::BOYS_ORDERS + m] = (double)boys_series_ld(m, (long double)T64);
+            b32[(size_t)i * md::BOYS_ORDERS + m] = (float)boys_series_ld(m, (long double)T32);
+        }
+    }
+
+    // ---- work-list regions: unscreened canonical quartets per class ----
+    e->reg.assign(N_COMBOS + 1, 0);
+    {
+        std::vector<std::vector<long long>> cnt(n_mol, std::vector<long long>(N_PAIR_CLASSES, 0));
+        for (int p = 0; p < e->n_pairs; ++p) cnt[e->pair_mol[p]][e->pair_pc[p]]++;
+        long long acc = 0;
+        for (int k = 0; k < N_COMBOS; ++k) {
+            e->reg[k] = acc;
+            const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+            for (int m = 0; m < n_mol; ++m)
+                acc += (p1 == p2) ? cnt[m][p1] * (cnt[m][p1] + 1) / 2 : cnt[m][p1] * cnt[m][p2];
+        }
+        e->reg[N_COMBOS] = acc;
+        e->list_cap = acc;
+    }
+
+    // ---- device upload ----
+    int rc;
+    if ((rc = upload(e.get(), e->d_pair_a, e->pair_a)) || (rc = upload(e.get(), e->d_pair_b, e->pair_b)) ||
+        (rc = upload(e.get(), e->d_pair_ps, e->pair_ps)) || (rc = upload(e.get(), e->d_pair_np, e->pair_np)) ||
+        (rc = upload(e.get(), e->d_pair_mol, e->pair_mol)) || (rc = upload(e.get(), e->d_pp64, pp64)) ||
+        (rc = upload(e.get(), e->d_pp32, pp32)) || (rc = upload(e.get(), e->d_sh_ao, e->sh_ao)) ||
+        (rc = upload(e.get(), e->d_sh_nao, e->sh_nao)) || (rc = upload(e.get(), e->d_sh_mn, e->sh_mn)) ||
+        (rc = upload(e.get(), e->d_sh_local, e->sh_local)) || (rc = upload(e.get(), e->d_sh_ns, e->sh_ns)) ||
+        (rc = upload(e.get(), e->d_sh_po, e->sh_po)) || (rc = upload(e.get(), e->d_sh_np, e->sh_np)) ||
+        (rc = upload(e.get(), e->d_sh_no, e->sh_no)) || (rc = upload(e.get(), e->d_sh_mol, e->sh_mol)) ||
+        (rc = upload(e.get(), e->d_sh_mbase, e->sh_mbase)) || (rc = upload(e.get(), e->d_sh_qbase, e->sh_qbase)) ||
+        (rc = upload(e.get(), e->d_sh_cf, e->sh_cf)) || (rc = upload(e.get(), e->d_ctr, e->sh_ctr)) ||
+        (rc = upload(e.get(), e->d_alpha, e->alpha)) || (rc = upload(e.get(), e->d_coeff, e->coeff)) ||
+        (rc = upload(e.get(), e->d_norm, e->norm)) || (rc = upload(e.get(), e->d_boys64, b64)) ||
+        (rc = upload(e.get(), e->d_boys32, b32)) || (rc = upload(e.get(), e->d_mbase, e->mol_mbase)) ||
+        (rc = upload(e.get(), e->d_mn, e->mol_N)) || (rc = upload(e.get(), e->d_reg, e->reg)))
+        return rc;
+    const long long ne = e->n_elems, nq = e->n_qelems;
+    e->d_Dt = e->mem.alloc<double>(ne);
+    e->d_Da = e->mem.alloc<double>(ne);
+    e->d_Db = e->mem.alloc<double>(ne);
+    e->d_Mt = e->mem.alloc<double>(nq);
+    e->d_Ma = e->mem.alloc<double>(nq);
+    e->d_Mb = e->mem.alloc<double>(nq);
+    e->d_AJ = e->mem.alloc<double>(ne);
+    e->d_AKa = e->mem.alloc<double>(ne);
+    e->d_AKb = e->mem.alloc<double>(ne);
+    e->d_J = e->mem.alloc<double>(ne);
+    e->d_Ka = e->mem.alloc<double>(ne);
+    e->d_Kb = e->mem.alloc<double>(ne);
+    e->d_list = e->mem.alloc<uint2>(e->list_cap);
+    e->d_cnt = e->mem.alloc<unsigned long long>(2 * N_COMBOS);
+    e->d_stats = e->mem.alloc<unsigned long long>(4 * N_COMBOS);
+    e->d_ranges = e->mem.alloc<long long>(4 * N_COMBOS);
+    if (!e->d_Dt || !e->d_Da || !e->d_Db || !e->d_Mt || !e->d_Ma || !e->d_Mb || !e->d_AJ || !e->d_AKa || !e->d_AKb ||
+        !e->d_J || !e->d_Ka || !e->d_Kb || !e->d_list || !e->d_cnt || !e->d_stats || !e->d_ranges)
+        return fail(HFX_ERR_NOMEM, "hipMalloc failed for the build buffers");
+
+    // ---- Schwarz table, then the class order it implies ----
+    {
+        // Class lists in creation order are enough for the diagonal quartets.
+        if ((rc = compute_schwarz(e.get()))) return rc;
+        if ((rc = sort_pairs(e.get(), e->own_q))) return rc;
+    }
+    *out = e.release();
+    return HFX_OK;
+}
+
+int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
+    if (!e || !args || !args->dD_tot || !args->dD_alpha || !J || !Ka)
+        return fail(HFX_ERR_INVALID, "hfx_jk: null argument");
+    const bool uhf = args->dD_beta != nullptr;
+    if (uhf && !Kb) return fail(HFX_ERR_INVALID, "hfx_jk: dD_beta given without Kb");
+    if (args->precision != HFX_FP64 && args->precision != HFX_MIXED)
+        return fail(HFX_ERR_INVALID, "hfx_jk: unknown precision");
+    const bool devp = (args->flags & HFX_JK_DEVICE_PTRS) != 0;
+    const long long ne = e->n_elems;
+    if (!devp) {
+        if (!finite_all(args->dD_tot, ne) || !finite_all(args->dD_alpha, ne) || (uhf && !finite_all(args->dD_beta, ne)))
+            return fail(HFX_ERR_NONFINITE, "hfx_jk: non-finite density");
+    }
+    if (args->Q) {
+        if (!finite_all(args->Q, e->n_qelems)) return fail(HFX_ERR_NONFINITE, "hfx_jk: non-finite Q");
+        if (e->sort_q.size() != (size_t)e->n_qelems ||
+            std::memcmp(e->sort_q.data(), args->Q, e->n_qelems * sizeof(double)) != 0) {
+            std::vector<double> q(args->Q, args->Q + e->n_qelems);
+            int rc = sort_pairs(e, q);
+            if (rc) return rc;
+        }
+    } else if (e->sort_q != e->own_q) {
+        int rc = sort_pairs(e, e->own_q);
+        if (rc) return rc;
+    }
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    const hipMemcpyKind h2d = devp ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind d2h = devp ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIPCHK(hipMemcpyAsync(e->d_Dt, args->dD_tot, ne * sizeof(double), h2d, s));
+    HIPCHK(hipMemcpyAsync(e->d_Da, args->dD_alpha, ne * sizeof(double), h2d, s));
+    if (uhf) HIPCHK(hipMemcpyAsync(e->d_Db, args->dD_beta, ne * sizeof(double), h2d, s));
+    HIPCHK(hipEventRecord(e->ev0, s));
+
+    // shell-pair density maxima
+    {
+        DmaxArgs d{};
+        d.pair_a = e->d_pair_a;
+        d.pair_b = e->d_pair_b;
+        d.sh_ao = e->d_sh_ao;
+        d.sh_nao = e->d_sh_nao;
+        d.sh_mbase = e->d_sh_mbase;
+        d.sh_mn = e->d_sh_mn;
+        d.sh_local = e->d_sh_local;
+        d.sh_qbase = e->d_sh_qbase;
+        d.sh_ns = e->d_sh_ns;
+        d.Dt = e->d_Dt;
+        d.Da = e->d_Da;
+        d.Db = uhf ? e->d_Db : nullptr;
+        d.Mt = e->d_Mt;
+        d.Ma = e->d_Ma;
+        d.Mb = e->d_Mb;
+        d.n_pairs = e->n_pairs;
+        hipLaunchKernelGGL(k_dmax, dim3((e->n_pairs + 255) / 256), dim3(256), 0, s, d);
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_cnt, 2 * N_COMBOS);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 4 * N_COMBOS);
+    const bool mixed = args->precision == HFX_MIXED && args->fp32_cut > 0.0;
+    for (int k = 0; k < N_COMBOS; ++k) {
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
+        ScreenArgs a{};
+        a.bra_ids = e->d_cls_ids[p1];
+        a.n_bra = (int)e->cls_ids[p1].size();
+        a.bra_pos0 = 0;
+        a.ket_ids = e->d_cls_ids[p2];
+        a.ket_q = e->d_cls_q[p2];
+        a.ket_seg_start = e->d_seg_start[p2];
+        a.ket_seg_end = e->d_seg_end[p2];
+        a.ket_molseg = e->d_molseg[p2];
+        a.diagonal = (p1 == p2);
+        a.pair_a = e->d_pair_a;
+        a.pair_b = e->d_pair_b;
+        a.pair_np = e->d_pair_np;
+        a.pair_mol = e->d_pair_mol;
+        a.pair_q = e->d_pair_q;
+        a.sh_local = e->d_sh_local;
+        a.sh_qbase = e->d_sh_qbase;
+        a.sh_ns = e->d_sh_ns;
+        a.Mt = e->d_Mt;
+        a.Ma = e->d_Ma;
+        a.Mb = uhf ? e->d_Mb : nullptr;
+        a.tauS = args->schwarz_thresh;
+        a.tauD = args->density_thresh;
+        a.cut = args->fp32_cut;
+        a.mixed = mixed;
+        a.nS_total = e->nS;
+        a.list = e->d_list;
+        a.reg_lo = e->reg[k];
+        a.reg_hi = e->reg[k + 1];
+        a.cnt = e->d_cnt + 2 * k;
+        a.stats = e->d_stats + 4 * k;
+        const int waves = a.n_bra;
+        hipLaunchKernelGGL(k_screen, dim3((waves + 3) / 4), dim3(256), 0, s, a);
+    }
+    hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_cnt, e->d_reg, e->d_ranges);
+    {
+        const int g = e->n_cu * 4;
+        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, ne, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, ne, 0.0);
+        if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, ne, 0.0);
+    }
+    EriArgs ea = eri_args(e);
+    ea.list = e->d_list;
+    ea.Dt = e->d_Dt;
+    ea.Da = e->d_Da;
+    ea.Db = uhf ? e->d_Db : nullptr;
+    ea.AJ = e->d_AJ;
+    ea.AKa = e->d_AKa;
+    ea.AKb = e->d_AKb;
+    ea.tauS = args->schwarz_thresh;
+    ea.out = nullptr;
+    ea.out_off = nullptr;
+    const int grid = eri_grid(e);
+    const bool timing = (args->flags & HFX_JK_CLASS_TIMING) != 0;
+    int ev_slot[2 * N_COMBOS];  // event index after each (class, precision) group, -1 if absent
+    int n_ev = 0;
+    if (timing) HIPCHK(hipEventRecord(e->cev[n_ev++], s));
+    for (int k = 0; k < N_COMBOS; ++k) {
+        ev_slot[2 * k] = ev_slot[2 * k + 1] = -1;
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
+        ea.range = e->d_ranges + 2 * k;
+        launch_eri(k, 0, ea, grid, s);
+        if (timing) {
+            HIPCHK(hipEventRecord(e->cev[n_ev], s));
+            ev_slot[2 * k] = n_ev++;
+        }
+        if (mixed) {
+            ea.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
+            launch_eri(k, 1, ea, grid, s);
+            if (timing) {
+                HIPCHK(hipEventRecord(e->cev[n_ev], s));
+                ev_slot[2 * k + 1] = n_ev++;
+            }
+        }
+    }
+    {
+        const dim3 g(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256))), e->n_mol);
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, e->d_J, e->d_mbase, e->d_mn);
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, e->d_Ka, e->d_mbase, e->d_mn);
+        if (uhf) hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, e->d_Kb, e->d_mbase, e->d_mn);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev1, s));
+    HIPCHK(hipMemcpyAsync(J, e->d_J, ne * sizeof(double), d2h, s));
+    HIPCHK(hipMemcpyAsync(Ka, e->d_Ka, ne * sizeof(double), d2h, s));
+    if (uhf) HIPCHK(hipMemcpyAsync(Kb, e->d_Kb, ne * sizeof(double), d2h, s));
+    unsigned long long cnt[2 * N_COMBOS], stv[4 * N_COMBOS];
+    const bool want_stats = st && !(args->flags & HFX_JK_NO_STATS);
+    if (want_stats) {
+        HIPCHK(hipMemcpyAsync(cnt, e->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(stv, e->d_stats, sizeof(stv), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        st->device_ms = ms;
+        if (want_stats) {
+            double flop = 0.0;
+            for (int k = 0; k < N_COMBOS; ++k) {
+                const long long c = (long long)(cnt[2 * k] + cnt[2 * k + 1]);
+                st->class_count[k] = c;
+                st->n_quartets += c;
+                st->n_fp32 += (long long)cnt[2 * k + 1];
+                st->hash_sum += stv[4 * k];
+                st->hash_xor ^= stv[4 * k + 1];
+                const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+                const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
+                const double f = model_flop_h(la, lb, lc, ld);
+                const double dig = 12.0 * ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+                st->class_gflop64[k] = ((double)stv[4 * k + 2] * f + dig * (double)cnt[2 * k]) * 1e-9;
+                st->class_gflop32[k] = ((double)stv[4 * k + 3] * f + dig * (double)cnt[2 * k + 1]) * 1e-9;
+                flop += st->class_gflop64[k] + st->class_gflop32[k];
+            }
+            st->model_gflop = flop;
+        }
+        if (timing) {
+            int prev = 0;
+            for (int x = 0; x < 2 * N_COMBOS; ++x) {
+                if (ev_slot[x] < 0) continue;
+                float cms = 0.f;
+                HIPCHK(hipEventElapsedTime(&cms, e->cev[prev], e->cev[ev_slot[x]]));
+                (x % 2 == 0 ? st->class_ms64 : st->class_ms32)[x / 2] = cms;
+                prev = ev_slot[x];
+            }
+        }
+    }
+    return HFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hfx_last_error(void) { return g_err.c_str(); }
+
+int hfx_create(const hfx_basis* basis, int device, hfx_engine** out) { return create_impl(1, basis, device, out); }
+
+int hfx_create_batched(int32_t n_mol, const hfx_basis* bases, int device, hfx_engine** out) {
+    return create_impl(n_mol, bases, device, out);
+}
+
+int hfx_destroy(hfx_engine* e) {
+    if (!e) return HFX_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    delete e;
+    return HFX_OK;
+}
+
+int hfx_sizes(const hfx_engine* e, int64_t* n_aos, int64_t* n_shells, int64_t* n_mol, int64_t* elems) {
+    if (!e) return fail(HFX_ERR_INVALID, "null engine");
+    if (n_aos) *n_aos = e->N_tot;
+    if (n_shells) *n_shells = e->nS;
+    if (n_mol) *n_mol = e->n_mol;
+    if (elems) *elems = e->n_elems;
+    return HFX_OK;
+}
+
+static int one_electron(hfx_engine* e, int kind, int32_t nat, const double* Z, const double* xyz,
+                        const int32_t* atom_mol, double* M) {
+    if (!e || !M) return fail(HFX_ERR_INVALID, "null argument");
+    if (kind == 2 && (nat < 0 || (nat > 0 && (!Z || !xyz)))) return fail(HFX_ERR_INVALID, "hfx_nuclear: bad atoms");
+    HIPCHK(hipSetDevice(e->device));
+    DevArena tmp;
+    double* dout = tmp.alloc<double>(e->n_elems);
+    if (!dout) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+    std::vector<double> az, axyz;
+    std::vector<int> atom0(e->n_mol + 1, 0);
+    if (kind == 2) {
+        if (e->n_mol > 1 && !atom_mol) return fail(HFX_ERR_INVALID, "hfx_nuclear: atom_mol required for batched engines");
+        for (int m = 0; m < e->n_mol; ++m) {
+            atom0[m] = (int)az.size();
+            for (int a = 0; a < nat; ++a)
+                if (!atom_mol || atom_mol[a] == m) {
+                    az.push_back(Z[a]);
+                    axyz.insert(axyz.end(), xyz + 3 * a, xyz + 3 * a + 3);
+                }
+        }
+        atom0[e->n_mol] = (int)az.size();
+        if (!finite_all(az.data(), (long long)az.size()) || !finite_all(axyz.data(), (long long)axyz.size()))
+            return fail(HFX_ERR_NONFINITE, "hfx_nuclear: non-finite atoms");
+    }
+    double* dz = tmp.alloc<double>(az.size());
+    double* dxyz = tmp.alloc<double>(axyz.size());
+    int* da0 = tmp.alloc<int>(atom0.size());
+    if (!dz || !dxyz || !da0) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+    if (!az.empty()) {
+        HIPCHK(hipMemcpy(dz, az.data(), az.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dxyz, axyz.data(), axyz.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(da0, atom0.data(), atom0.size() * sizeof(int), hipMemcpyHostToDevice));
+    OneArgs o{};
+    o.pair_a = e->d_pair_a;
+    o.pair_b = e->d_pair_b;
+    o.sh_po = e->d_sh_po;
+    o.sh_np = e->d_sh_np;
+    o.sh_no = e->d_sh_no;
+    o.sh_ao = e->d_sh_ao;
+    o.sh_mbase = e->d_sh_mbase;
+    o.sh_mn = e->d_sh_mn;
+    o.sh_mol = e->d_sh_mol;
+    o.ctr = e->d_ctr;
+    o.alpha = e->d_alpha;
+    o.coeff = e->d_coeff;
+    o.norm = e->d_norm;
+    o.atom_z = dz;
+    o.atom_xyz = dxyz;
+    o.mol_atom0 = da0;
+    o.boys = e->d_boys64;
+    o.out = dout;
+    for (int pc = 0; pc < N_PAIR_CLASSES; ++pc) {
+        o.ids = e->d_cls_ids[pc];
+        o.n = (int)e->cls_ids[pc].size();
+        if (kind == 0) launch_one<0>(pc, o, e->stream);
+        else if (kind == 1) launch_one<1>(pc, o, e->stream);
+        else launch_one<2>(pc, o, e->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(M, dout, e->n_elems * sizeof(double), hipMemcpyDeviceToHost));
+    return HFX_OK;
+}
+
+int hfx_overlap(hfx_engine* e, double* S) { return one_electron(e, 0, 0, nullptr, nullptr, nullptr, S); }
+int hfx_kinetic(hfx_engine* e, double* T) { return one_electron(e, 1, 0, nullptr, nullptr, nullptr, T); }
+int hfx_nuclear(hfx_engine* e, int32_t nat, const double* Z, const double* xyz, const int32_t* atom_mol, double* V) {
+    return one_electron(e, 2, nat, Z, xyz, atom_mol, V);
+}
+
+int hfx_schwarz(hfx_engine* e, double* Q) {
+    if (!e || !Q) return fail(HFX_ERR_INVALID, "null argument");
+    std::memcpy(Q, e->own_q.data(), e->n_qelems * sizeof(double));
+    return HFX_OK;
+}
+
+int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_t* offsets, double* out) {
+    if (!e || e->n_mol != 1 || (n > 0 && (!abcd || !offsets || !out)))
+        return fail(HFX_ERR_INVALID, "hfx_eri_quartets: bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    struct Req {
+        long long q;
+        bool swap_bk, swap_ab, swap_cd;
+    };
+    std::vector<std::vector<uint2>> ents(N_COMBOS);
+    std::vector<std::vector<Req>> reqs(N_COMBOS);
+    for (long long q = 0; q < n; ++q) {
+        const int* s = &abcd[4 * q];
+        for (int k = 0; k < 4; ++k)
+            if (s[k] < 0 || s[k] >= e->nS) return fail(HFX_ERR_INVALID, "hfx_eri_quartets: shell index out of range");
+        const int p1 = e->pair_of.at((long long)std::min(s[0], s[1]) * e->nS + std::max(s[0], s[1]));
+        const int p2 = e->pair_of.at((long long)std::min(s[2], s[3]) * e->nS + std::max(s[2], s[3]));
+        int bra = p1, ket = p2;
+        bool swap_bk = false;
+        if (e->pair_pc[p1] < e->pair_pc[p2]) {
+            std::swap(bra, ket);
+            swap_bk = true;
+        }
+        const int k = combo_of(e->pair_pc[bra], e->pair_pc[ket]);
+        Req r;
+        r.q = q;
+        r.swap_bk = swap_bk;
+        // Does the requested (a,b) [(c,d)] order differ from the pair's stored orientation?
+        r.swap_ab = e->pair_a[p1] != s[0];
+        r.swap_cd = e->pair_a[p2] != s[2];
+        ents[k].push_back(make_uint2(bra, ket));
+        reqs[k].push_back(r);
+    }
+    for (int k = 0; k < N_COMBOS; ++k) {
+        if (ents[k].empty()) continue;
+        std::vector<double> blk;
+        std::vector<long long> offs;
+        int rc = store_blocks(e, k, ents[k], blk, offs);
+        if (rc) return rc;
+        for (size_t x = 0; x < ents[k].size(); ++x) {
+            const Req& r = reqs[k][x];
+            const int* s = &abcd[4 * r.q];
+            const int na = ncart_h(e->sh_l[s[0]]), nb = ncart_h(e->sh_l[s[1]]), nc = ncart_h(e->sh_l[s[2]]),
+                      nd = ncart_h(e->sh_l[s[3]]);
+            // stored block: [bra pair comps][ket pair comps] in the pairs' own orientation
+            const int bp = ents[k][x].x, kp = ents[k][x].y;
+            const int nbk = ncart_h(e->sh_l[e->pair_a[kp]]) * ncart_h(e->sh_l[e->pair_b[kp]]);
+            const int nkb = ncart_h(e->sh_l[e->pair_b[bp]]);
+            const int nkd = ncart_h(e->sh_l[e->pair_b[kp]]);
+            const bool same = (s[0] == s[2] && s[1] == s[3]);
+            double* dst = out + offsets[r.q];
+            for (int A = 0; A < na; ++A)
+                for (int B = 0; B < nb; ++B)
+                    for (int Cc = 0; Cc < nc; ++Cc)
+                        for (int D = 0; D < nd; ++D) {
+                            const long long i = e->sh_ao[s[0]] + A, j = e->sh_ao[s[1]] + B;
+                            const long long kk = e->sh_ao[s[2]] + Cc, l = e->sh_ao[s[3]] + D;
+                            double v = 0.0;
+                            const bool zero = (j > i) || (l > kk) || (same && i * (i + 1) / 2 + j < kk * (kk + 1) / 2 + l);
+                            if (!zero) {
+                                // components in the pair orientations
+                                const int p1a = r.swap_ab ? B : A, p1b = r.swap_ab ? A : B;
+                                const int p2a = r.swap_cd ? D : Cc, p2b = r.swap_cd ? Cc : D;
+                                int ba, bb, ka, kb;
+                                if (!r.swap_bk) {
+                                    ba = p1a; bb = p1b; ka = p2a; kb = p2b;
+                                } else {
+                                    ba = p2a; bb = p2b; ka = p1a; kb = p1b;
+                                }
+                                v = blk[offs[x] + (long long)(ba * nkb + bb) * nbk + (ka * nkd + kb)];
+                            }
+                            dst[((long long)A * nb + B) * nc * nd + (long long)Cc * nd + D] = v;
+                        }
+        }
+    }
+    return HFX_OK;
+}
+
+int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
+    return jk_impl(e, args, J, Ka, Kb, st);
+}
+
+}  // extern "C"

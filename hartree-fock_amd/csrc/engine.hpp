@@ -1,0 +1,82 @@
+// engine.hpp - device-side data layout shared by the engine and its kernel translation units.
+//
+// HBM layout (all arrays allocated once per engine):
+//   shells       sh_ao (AO offset inside its molecule), sh_mbase (offset of the molecule's
+//                N_m x N_m block in every packed matrix), sh_mn (N_m), sh_cf[6*s + c]
+//                (Cartesian normalization ratio N(p,c)/N(p,0), independent of p)
+//   shell pairs  pair_a, pair_b (la >= lb; a >= b when la == lb), pair_ps/pair_np (range of
+//                the pair's primitive pairs)
+//   prim pairs   PrimPair64 / PrimPair32, 96 / 48 bytes each, one per primitive pair of
+//                every shell pair, pair-major
+//   work lists   uint2 {bra pair, ket pair} per surviving shell quartet, one region per
+//                quartet class: FP64 entries fill the region from its start, FP32 entries
+//                from its end
+//   matrices     N x N column-major blocks, one per molecule, concatenated
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hfx {
+
+// Everything a primitive pair contributes to an ERI.
+//   kp = K_ab c_a c_b g_a g_b / p with K_ab = exp(-ab/p |A-B|^2) (src/IntegralEngine.cpp:1006-1007),
+//   c the contraction coefficients and g the primitive part of the normalization.
+template <class R>
+struct PrimPair {
+    R p, px, py, pz, kp, pax, pay, paz, pbx, pby, pbz, oo2p;
+};
+using PrimPair64 = PrimPair<double>;
+using PrimPair32 = PrimPair<float>;
+
+// Pair class of (la >= lb) and quartet class of (pair class pc1 >= pc2).
+__host__ __device__ constexpr int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
+__host__ __device__ constexpr int combo_of(int pc1, int pc2) { return pc1 * (pc1 + 1) / 2 + pc2; }
+__host__ __device__ constexpr int combo_pc1(int k) {
+    int p = 0;
+    while ((p + 1) * (p + 2) / 2 <= k) ++p;
+    return p;
+}
+__host__ __device__ constexpr int combo_pc2(int k) { return k - combo_pc1(k) * (combo_pc1(k) + 1) / 2; }
+__host__ __device__ constexpr int pc_la(int pc) { return combo_pc1(pc); }
+__host__ __device__ constexpr int pc_lb(int pc) { return combo_pc2(pc); }
+
+constexpr int N_PAIR_CLASSES = 6;
+constexpr int N_COMBOS = 21;
+
+struct EriArgs {
+    const int* pair_a;
+    const int* pair_b;
+    const int* pair_ps;
+    const int* pair_np;
+    const PrimPair64* pp64;
+    const PrimPair32* pp32;
+    const int* sh_ao;
+    const long long* sh_mbase;
+    const int* sh_mn;
+    const double* sh_cf;
+    const uint2* list;
+    const long long* range;  // [lo, hi) of this class's entries in list
+    const double* Dt;
+    const double* Da;
+    const double* Db;  // NULL: no K_beta
+    double* AJ;
+    double* AKa;
+    double* AKb;
+    double tauS;
+    double* out;               // MODE_STORE: blocks [ab][cd], canonical class order
+    const long long* out_off;  // MODE_STORE: offset of each entry's block
+    const double* boys64;
+    const float* boys32;
+};
+
+// Each kernel translation unit compiles chunks [begin, end) of one class and registers a
+// launcher for them at load time; launch_eri runs every registered part of a class.
+using EriPartFn = void (*)(int fp32, const EriArgs& a, int grid, hipStream_t s);
+bool register_eri_part(int combo, int begin, int end, int nch, EriPartFn fn);
+// Number of chunk kernels of a class, or -1 when its parts do not cover [0, nch).
+int eri_chunks(int combo);
+void launch_eri(int combo, int fp32, const EriArgs& a, int grid, hipStream_t s);
+
+}  // namespace hfx

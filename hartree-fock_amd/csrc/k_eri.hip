@@ -1,0 +1,340 @@
+// k_eri.hip - contracted ERIs of one quartet class, digested into J/K on the fly.
+//
+// Compiled once per class with -DHFX_COMBO=k (k = pc1(pc1+1)/2 + pc2, pair class
+// pc = la(la+1)/2 + lb), so the 21 classes build in parallel.
+//
+// One thread owns one shell quartet (bra pair, ket pair) from the class's work list and a
+// chunk of CB bra Cartesian components; chunks are separate launches so that every array
+// index is a compile-time constant.  Per primitive quartet the thread evaluates the Boys
+// function (table slice in LDS), the Hermite R^0_{tuv} (scaled by the prefactor), the bra
+// contraction Y_{t'u'v'} = sum_tuv E^{ab}_tuv R_{t+t',u+u',v+v'} for each of its bra
+// components, and accumulates sum_{t'u'v'} (-1)^{t'+u'+v'} E^{cd}_{t'u'v'} Y_{t'u'v'} into
+// the contracted integrals.  This is the primitive integral of the reference's
+// electronRepulsion (src/IntegralEngine.cpp:897-934) in factored order.
+//
+// Digestion: the full block of the quartet is used with the permutational degeneracy
+// s = (a==b ? 1/2 : 1)(c==d ? 1/2 : 1)(ab==cd ? 1/2 : 1); the integrals below tau_S are
+// dropped as the reference's AO test (src/SCF.cpp:670-671).  Contributions go to
+//   AJ[i,j] += 2 s I D[k,l],  AJ[k,l] += 2 s I D[i,j],
+//   AK[i,k] += s I D[j,l], AK[i,l] += s I D[j,k], AK[j,k] += s I D[i,l], AK[j,l] += s I D[i,k]
+// accumulated per thread in registers and flushed with FP64 atomics; J = AJ + AJ^T and
+// K = AK + AK^T afterwards.  This equals the reference's canonical-AO digestion
+// (src/SCF.cpp:633-711) summed over the 8 permutations.
+#include "engine.hpp"
+#include "md.hpp"
+
+#ifndef HFX_COMBO
+#error "HFX_COMBO must be defined"
+#endif
+
+#define HFX_CAT2(a, b) a##b
+#define HFX_CAT(a, b) HFX_CAT2(a, b)
+
+namespace hfx {
+namespace {
+
+using namespace md;
+
+constexpr int K_COMBO = HFX_COMBO;
+constexpr int PC1 = combo_pc1(K_COMBO), PC2 = combo_pc2(K_COMBO);
+constexpr int LA = pc_la(PC1), LB = pc_lb(PC1), LC = pc_la(PC2), LD = pc_lb(PC2);
+constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+constexpr int NAB = NA * NB, NCD = NC * ND;
+constexpr int ACC_MAX = 36;
+constexpr int CB_MAX = cmax(1, ACC_MAX / NCD);
+constexpr int NCH = (NAB + CB_MAX - 1) / CB_MAX;
+constexpr int CB = (NAB + NCH - 1) / NCH;
+constexpr int LTOT = LA + LB + LC + LD;
+constexpr int LCD = LC + LD;
+constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
+
+template <class R>
+__device__ __forceinline__ const PrimPair<R>* pp_of(const EriArgs& a);
+template <>
+__device__ __forceinline__ const PrimPair<double>* pp_of<double>(const EriArgs& a) { return a.pp64; }
+template <>
+__device__ __forceinline__ const PrimPair<float>* pp_of<float>(const EriArgs& a) { return a.pp32; }
+template <class R>
+__device__ __forceinline__ const R* boys_of(const EriArgs& a);
+template <>
+__device__ __forceinline__ const double* boys_of<double>(const EriArgs& a) { return a.boys64; }
+template <>
+__device__ __forceinline__ const float* boys_of<float>(const EriArgs& a) { return a.boys32; }
+
+__device__ __forceinline__ void atomic_add(double* p, double v) {
+    if (v != 0.0) unsafeAtomicAdd(p, v);
+}
+
+// Is bra component row ia (of shell a) / ib (of shell b) touched by chunk C0?
+__host__ __device__ constexpr bool chunk_has_a(int c0, int ia) {
+    for (int ab = c0; ab < c0 + CB && ab < NAB; ++ab)
+        if (ab / NB == ia) return true;
+    return false;
+}
+__host__ __device__ constexpr bool chunk_has_b(int c0, int ib) {
+    for (int ab = c0; ab < c0 + CB && ab < NAB; ++ab)
+        if (ab % NB == ib) return true;
+    return false;
+}
+
+template <class R, int C0>
+__device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R* tab) {
+    const uint2 e = a.list[qi];
+    const int pid = (int)e.x, qid = (int)e.y;
+    const int ps = a.pair_ps[pid], np = a.pair_np[pid];
+    const int qs = a.pair_ps[qid], nq = a.pair_np[qid];
+    const PrimPair<R>* __restrict__ pp = pp_of<R>(a);
+
+    R acc[CB][NCD];
+    sfor<CB>([&](auto c) { sfor<NCD>([&](auto d) { acc[c][d] = R(0); }); });
+
+    for (int ip = 0; ip < np; ++ip) {
+        const PrimPair<R> A = pp[ps + ip];
+        Ecoef<LA, LB, R> Ex, Ey, Ez;
+        hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
+        hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
+        hermite_e<LA, LB>(A.oo2p, A.paz, A.pbz, Ez);
+        for (int iq = 0; iq < nq; ++iq) {
+            const PrimPair<R> B = pp[qs + iq];
+            const R pq = A.p + B.p;
+            const R rs = R(1) / dev_sqrt(pq);
+            const R alpha = (A.p * B.p) * (rs * rs);
+            const R X = A.px - B.px, Y = A.py - B.py, Z = A.pz - B.pz;
+            const R T = alpha * (X * X + Y * Y + Z * Z);
+            const R pref = R(TWO_PI_2_5) * rs * A.kp * B.kp;
+            R F[LTOT + 1];
+            boys<LTOT>(T, tab, F);
+            R base[LTOT + 1];
+            {
+                const R m2a = R(-2) * alpha;
+                R pw = pref;
+                sfor<LTOT + 1>([&](auto n) {
+                    base[n] = pw * F[n];
+                    if constexpr (decltype(n)::value < LTOT) pw *= m2a;
+                });
+            }
+            R Rt[nherm(LTOT)];
+            hermite_r<LTOT>(base, X, Y, Z, Rt);
+
+            Ecoef<LC, LD, R> Fx, Fy, Fz;
+            hermite_e<LC, LD>(B.oo2p, B.pax, B.pbx, Fx);
+            hermite_e<LC, LD>(B.oo2p, B.pay, B.pby, Fy);
+            hermite_e<LC, LD>(B.oo2p, B.paz, B.pbz, Fz);
+
+            sfor<CB>([&](auto c_) {
+                constexpr int c = decltype(c_)::value;
+                constexpr int ab = C0 + c;
+                if constexpr (ab < NAB) {
+                    constexpr int ia = ab / NB, ib = ab % NB;
+                    constexpr int ax = cart_i(LA, ia), ay = cart_j(LA, ia), az = cart_k(LA, ia);
+                    constexpr int bx = cart_i(LB, ib), by = cart_j(LB, ib), bz = cart_k(LB, ib);
+                    R Yv[nherm(LCD)];
+                    sfor<nherm(LCD)>([&](auto h_) {
+                        constexpr int h = decltype(h_)::value;
+                        constexpr int t2 = h_t(h), u2 = h_u(h), v2 = h_v(h);
+                        Yv[h] = sum_of<az + bz + 1>([&](auto v_) {
+                            constexpr int v = decltype(v_)::value;
+                            return Ez.e[az][bz][v] * sum_of<ay + by + 1>([&](auto u_) {
+                                       constexpr int u = decltype(u_)::value;
+                                       return Ey.e[ay][by][u] * sum_of<ax + bx + 1>([&](auto t_) {
+                                                  constexpr int t = decltype(t_)::value;
+                                                  return Ex.e[ax][bx][t] * Rt[hidx(t + t2, u + u2, v + v2)];
+                                              });
+                                   });
+                        });
+                    });
+                    sfor<NCD>([&](auto cd_) {
+                        constexpr int cd = decltype(cd_)::value;
+                        constexpr int jc = cd / ND, jd = cd % ND;
+                        constexpr int cx = cart_i(LC, jc), cy = cart_j(LC, jc), cz = cart_k(LC, jc);
+                        constexpr int dx = cart_i(LD, jd), dy = cart_j(LD, jd), dz = cart_k(LD, jd);
+                        acc[c][cd] += sum_of<cz + dz + 1>([&](auto v_) {
+                            constexpr int v = decltype(v_)::value;
+                            return Fz.e[cz][dz][v] * sum_of<cy + dy + 1>([&](auto u_) {
+                                       constexpr int u = decltype(u_)::value;
+                                       return Fy.e[cy][dy][u] * sum_of<cx + dx + 1>([&](auto t_) {
+                                                  constexpr int t = decltype(t_)::value;
+                                                  constexpr R sg = ((t + u + v) & 1) ? R(-1) : R(1);
+                                                  return (sg * Fx.e[cx][dx][t]) * Yv[hidx(t, u, v)];
+                                              });
+                                   });
+                        });
+                    });
+                }
+            });
+        }
+    }
+
+    const int sa = a.pair_a[pid], sb = a.pair_b[pid], sc = a.pair_a[qid], sd = a.pair_b[qid];
+    double cfa[NA], cfb[NB], cfc[NC], cfd[ND];
+    sfor<NA>([&](auto x) { cfa[x] = a.sh_cf[6 * sa + x]; });
+    sfor<NB>([&](auto x) { cfb[x] = a.sh_cf[6 * sb + x]; });
+    sfor<NC>([&](auto x) { cfc[x] = a.sh_cf[6 * sc + x]; });
+    sfor<ND>([&](auto x) { cfd[x] = a.sh_cf[6 * sd + x]; });
+
+    if (a.out) {  // store mode: normalized block, no screening
+        double* out = a.out + a.out_off[qi];
+        sfor<CB>([&](auto c_) {
+            constexpr int ab = C0 + decltype(c_)::value;
+            if constexpr (ab < NAB) {
+                sfor<NCD>([&](auto cd_) {
+                    constexpr int cd = decltype(cd_)::value;
+                    out[ab * NCD + cd] = (double)acc[decltype(c_)::value][cd] * (cfa[ab / NB] * cfb[ab % NB]) *
+                                         (cfc[cd / ND] * cfd[cd % ND]);
+                });
+            }
+        });
+        return;
+    } else {
+        const long long base = a.sh_mbase[sa];
+        const long long N = a.sh_mn[sa];
+        const int oA = a.sh_ao[sa], oB = a.sh_ao[sb], oC = a.sh_ao[sc], oD = a.sh_ao[sd];
+        double s = 1.0;
+        if (sa == sb) s *= 0.5;
+        if (sc == sd) s *= 0.5;
+        if (pid == qid) s *= 0.5;
+        const double tauS = a.tauS;
+        const double* __restrict__ Dt = a.Dt + base;
+        const double* __restrict__ Da = a.Da + base;
+        const double* __restrict__ Db = a.Db ? a.Db + base : nullptr;
+
+        // Screened, degeneracy-scaled integral of bra component ab (chunk slot c) and ket cd.
+        auto val = [&](auto c_, auto cd_) -> double {
+            constexpr int c = decltype(c_)::value, cd = decltype(cd_)::value;
+            constexpr int ab = C0 + c;
+            const double v = (double)acc[c][cd] * (cfa[ab / NB] * cfb[ab % NB]) * (cfc[cd / ND] * cfd[cd % ND]);
+            return (fabs(v) < tauS) ? 0.0 : v * s;
+        };
+        double* AJ = a.AJ + base;
+        {
+            double Jab[CB], Jcd[NCD];
+            sfor<CB>([&](auto x) { Jab[x] = 0.0; });
+            sfor<NCD>([&](auto x) { Jcd[x] = 0.0; });
+            sfor<NCD>([&](auto cd_) {
+                constexpr int cd = decltype(cd_)::value;
+                const long long k = oC + cd / ND, l = oD + cd % ND;
+                const double dkl = Dt[k + l * N];
+                sfor<CB>([&](auto c_) {
+                    constexpr int c = decltype(c_)::value;
+                    constexpr int ab = C0 + c;
+                    if constexpr (ab < NAB) {
+                        const long long i = oA + ab / NB, j = oB + ab % NB;
+                        const double v = val(c_, cd_);
+                        Jab[c] += v * dkl;
+                        Jcd[cd] += v * Dt[i + j * N];
+                    }
+                });
+            });
+            sfor<CB>([&](auto c_) {
+                constexpr int c = decltype(c_)::value;
+                constexpr int ab = C0 + c;
+                if constexpr (ab < NAB) atomic_add(&AJ[(oA + ab / NB) + (oB + ab % NB) * N], 2.0 * Jab[c]);
+            });
+            sfor<NCD>([&](auto cd_) {
+                constexpr int cd = decltype(cd_)::value;
+                atomic_add(&AJ[(oC + cd / ND) + (oD + cd % ND) * N], 2.0 * Jcd[cd]);
+            });
+        }
+        // Exchange with one spin density Dx into the accumulator AKx.
+        auto exchange = [&](const double* __restrict__ Dx, double* AKx) {
+            double Kac[NA][NC], Kad[NA][ND], Kbc[NB][NC], Kbd[NB][ND];
+            sfor<NA>([&](auto x) {
+                sfor<NC>([&](auto y) { Kac[x][y] = 0.0; });
+                sfor<ND>([&](auto y) { Kad[x][y] = 0.0; });
+            });
+            sfor<NB>([&](auto x) {
+                sfor<NC>([&](auto y) { Kbc[x][y] = 0.0; });
+                sfor<ND>([&](auto y) { Kbd[x][y] = 0.0; });
+            });
+            sfor<NCD>([&](auto cd_) {
+                constexpr int cd = decltype(cd_)::value;
+                constexpr int jc = cd / ND, jd = cd % ND;
+                const long long k = oC + jc, l = oD + jd;
+                sfor<CB>([&](auto c_) {
+                    constexpr int c = decltype(c_)::value;
+                    constexpr int ab = C0 + c;
+                    if constexpr (ab < NAB) {
+                        constexpr int ia = ab / NB, ib = ab % NB;
+                        const long long i = oA + ia, j = oB + ib;
+                        const double v = val(c_, cd_);
+                        Kac[ia][jc] += v * Dx[j + l * N];
+                        Kad[ia][jd] += v * Dx[j + k * N];
+                        Kbc[ib][jc] += v * Dx[i + l * N];
+                        Kbd[ib][jd] += v * Dx[i + k * N];
+                    }
+                });
+            });
+            sfor<NA>([&](auto ia_) {
+                constexpr int ia = decltype(ia_)::value;
+                if constexpr (chunk_has_a(C0, ia)) {
+                    sfor<NC>([&](auto jc_) {
+                        constexpr int jc = decltype(jc_)::value;
+                        atomic_add(&AKx[(oA + ia) + (oC + jc) * N], Kac[ia][jc]);
+                    });
+                    sfor<ND>([&](auto jd_) {
+                        constexpr int jd = decltype(jd_)::value;
+                        atomic_add(&AKx[(oA + ia) + (oD + jd) * N], Kad[ia][jd]);
+                    });
+                }
+            });
+            sfor<NB>([&](auto ib_) {
+                constexpr int ib = decltype(ib_)::value;
+                if constexpr (chunk_has_b(C0, ib)) {
+                    sfor<NC>([&](auto jc_) {
+                        constexpr int jc = decltype(jc_)::value;
+                        atomic_add(&AKx[(oB + ib) + (oC + jc) * N], Kbc[ib][jc]);
+                    });
+                    sfor<ND>([&](auto jd_) {
+                        constexpr int jd = decltype(jd_)::value;
+                        atomic_add(&AKx[(oB + ib) + (oD + jd) * N], Kbd[ib][jd]);
+                    });
+                }
+            });
+        };
+        exchange(Da, a.AKa + base);
+        if (Db) exchange(Db, a.AKb + base);
+    }
+}
+
+template <class R, int C0>
+__global__ __launch_bounds__(256) void k_eri(EriArgs a) {
+    using S = BoysSlice<LTOT, R>;
+    __shared__ R tab[BOYS_POINTS * S::nord];
+    const long long lo = a.range[0], hi = a.range[1];
+    const long long n = hi - lo;
+    const long long first = (long long)blockIdx.x * blockDim.x;
+    if (first >= n) return;  // the whole block has no work: uniform exit before the barrier
+    load_boys<LTOT, R>(boys_of<R>(a), tab);
+    __syncthreads();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long w = first + threadIdx.x; w < n; w += stride) quartet<R, C0>(a, lo + w, tab);
+}
+
+#ifndef HFX_CHUNK_BEGIN
+#define HFX_CHUNK_BEGIN 0
+#endif
+#ifndef HFX_CHUNK_END
+#define HFX_CHUNK_END 1000
+#endif
+constexpr int CH_BEGIN = HFX_CHUNK_BEGIN;
+constexpr int CH_END = HFX_CHUNK_END < NCH ? HFX_CHUNK_END : NCH;
+
+template <class R>
+void launch_part(const EriArgs& a, int grid, hipStream_t s) {
+    sfor_host<(CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0)>([&](auto ch) {
+        constexpr int C0 = (CH_BEGIN + decltype(ch)::value) * CB;
+        hipLaunchKernelGGL((k_eri<R, C0>), dim3(grid), dim3(256), 0, s, a);
+    });
+}
+
+void launch_this_part(int fp32, const EriArgs& a, int grid, hipStream_t s) {
+    if (fp32)
+        launch_part<float>(a, grid, s);
+    else
+        launch_part<double>(a, grid, s);
+}
+
+const bool registered = register_eri_part(K_COMBO, CH_BEGIN, CH_END, NCH, &launch_this_part);
+
+}  // namespace
+}  // namespace hfx

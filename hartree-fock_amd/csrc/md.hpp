@@ -1,0 +1,275 @@
+// md.hpp - McMurchie-Davidson building blocks for CDNA4, fully unrolled at compile time.
+//
+// Everything here is templated on the angular momenta so that every array is indexed by a
+// constant expression and lives in VGPRs:
+//   hermite_e   1-D Hermite expansion coefficients E^{ij}_t of a Gaussian product
+//               (the recursion of the reference's computeHermiteCoeffs,
+//               src/IntegralEngine.cpp:1025-1079, restated per direction)
+//   hermite_r   Hermite Coulomb integrals R^0_{tuv}, built in place level by level from
+//               R^n_{000} = (-2 alpha)^n F_n(T) (src/IntegralEngine.cpp:1082-1162)
+//   boys        F_m(T), m = 0..L: asymptotic form above T = 30 as the reference
+//               (src/Boys.cpp:16-23); below, an NT-term Taylor expansion around the nearest
+//               point of a 0.1-spaced table (kept in LDS), then downward recursion.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
+
+namespace md {
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <class F, int... Is>
+__device__ __forceinline__ void sfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(ic<Is>{}), ...);
+}
+// Calls f(ic<0>), ..., f(ic<N-1>) in order.
+template <int N, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (N > 0) sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int tri(int m) { return m * (m + 1) / 2; }
+__host__ __device__ constexpr int tet(int s) { return s * (s + 1) * (s + 2) / 6; }
+__host__ __device__ constexpr int nherm(int L) { return tet(L + 1); }
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// Cartesian exponents of component c of a shell of angular momentum l, in the reference's
+// AO order (src/Basis.cpp:47-59): i from l down to 0, j from l-i down to 0, k = l-i-j.
+__host__ __device__ constexpr int cart_i(int l, int c) {
+    int idx = 0;
+    for (int i = l; i >= 0; --i)
+        for (int j = l - i; j >= 0; --j) {
+            if (idx == c) return i;
+            ++idx;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int cart_j(int l, int c) {
+    int idx = 0;
+    for (int i = l; i >= 0; --i)
+        for (int j = l - i; j >= 0; --j) {
+            if (idx == c) return j;
+            ++idx;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int cart_k(int l, int c) { return l - cart_i(l, c) - cart_j(l, c); }
+
+// Packed Hermite index of (t,u,v): all triples of total s = t+u+v follow those of total s-1.
+__host__ __device__ constexpr int hidx(int t, int u, int v) { return tet(t + u + v) + tri(u + v) + v; }
+// Inverse within one level s: entry k has u+v = m with tri(m) <= k < tri(m+1).
+__host__ __device__ constexpr int lev_m(int k) {
+    int m = 0;
+    while (tri(m + 1) <= k) ++m;
+    return m;
+}
+// Full inverse of hidx.
+__host__ __device__ constexpr int h_level(int h) {
+    int s = 0;
+    while (tet(s + 1) <= h) ++s;
+    return s;
+}
+__host__ __device__ constexpr int h_v(int h) { return (h - tet(h_level(h))) - tri(lev_m(h - tet(h_level(h)))); }
+__host__ __device__ constexpr int h_u(int h) { return lev_m(h - tet(h_level(h))) - h_v(h); }
+__host__ __device__ constexpr int h_t(int h) { return h_level(h) - lev_m(h - tet(h_level(h))); }
+
+// sum_{k<N} term(ic<k>), starting from the first term (no added zero).
+template <int N, class F>
+__device__ __forceinline__ auto sum_of(F&& term) {
+    auto s = term(ic<0>{});
+    sfor<N - 1>([&](auto K_) { s += term(ic<decltype(K_)::value + 1>{}); });
+    return s;
+}
+
+// E[i][j][t] for i <= LA, j <= LB, t <= i+j, one Cartesian direction.
+template <int LA, int LB, class R>
+struct Ecoef {
+    R e[LA + 1][LB + 1][LA + LB + 1];
+};
+
+template <int LA, int LB, class R>
+__device__ __forceinline__ void hermite_e(R oo2p, R PA, R PB, Ecoef<LA, LB, R>& E) {
+    E.e[0][0][0] = R(1);
+    sfor<LA>([&](auto I_) {
+        constexpr int i = decltype(I_)::value + 1;
+        sfor<i + 1>([&](auto T_) {
+            constexpr int t = decltype(T_)::value;
+            R v;
+            if constexpr (t <= i - 1) {
+                v = PA * E.e[i - 1][0][t];
+                if constexpr (t >= 1) v += oo2p * E.e[i - 1][0][t - 1];
+            } else {
+                v = oo2p * E.e[i - 1][0][t - 1];
+            }
+            if constexpr (t + 1 <= i - 1) v += R(t + 1) * E.e[i - 1][0][t + 1];
+            E.e[i][0][t] = v;
+        });
+    });
+    sfor<LB>([&](auto J_) {
+        constexpr int j = decltype(J_)::value + 1;
+        sfor<LA + 1>([&](auto I_) {
+            constexpr int i = decltype(I_)::value;
+            constexpr int top = i + j;
+            sfor<top + 1>([&](auto T_) {
+                constexpr int t = decltype(T_)::value;
+                R v;
+                if constexpr (t <= top - 1) {
+                    v = PB * E.e[i][j - 1][t];
+                    if constexpr (t >= 1) v += oo2p * E.e[i][j - 1][t - 1];
+                } else {
+                    v = oo2p * E.e[i][j - 1][t - 1];
+                }
+                if constexpr (t + 1 <= top - 1) v += R(t + 1) * E.e[i][j - 1][t + 1];
+                E.e[i][j][t] = v;
+            });
+        });
+    });
+}
+
+// R[hidx(t,u,v)] = R^0_{tuv} for t+u+v <= L, given base[n] = scale * (-2 alpha)^n F_n(T).
+template <int L, class R>
+__device__ __forceinline__ void hermite_r(const R (&base)[L + 1], R X, R Y, R Z, R (&Rt)[nherm(L)]) {
+    Rt[0] = base[L];
+    sfor<L>([&](auto N_) {
+        constexpr int n = L - 1 - decltype(N_)::value;
+        sfor<L - n>([&](auto S_) {
+            constexpr int s = L - n - decltype(S_)::value;  // level s = L-n, ..., 1
+            sfor<tri(s + 1)>([&](auto K_) {
+                constexpr int k = decltype(K_)::value;
+                constexpr int m = lev_m(k);
+                constexpr int v = k - tri(m);
+                constexpr int u = m - v;
+                constexpr int t = s - m;
+                R val;
+                if constexpr (t > 0) {
+                    val = X * Rt[hidx(t - 1, u, v)];
+                    if constexpr (t > 1) val += R(t - 1) * Rt[hidx(t - 2, u, v)];
+                } else if constexpr (u > 0) {
+                    val = Y * Rt[hidx(t, u - 1, v)];
+                    if constexpr (u > 1) val += R(u - 1) * Rt[hidx(t, u - 2, v)];
+                } else {
+                    val = Z * Rt[hidx(t, u, v - 1)];
+                    if constexpr (v > 1) val += R(v - 1) * Rt[hidx(t, u, v - 2)];
+                }
+                Rt[hidx(t, u, v)] = val;
+            });
+        });
+        Rt[0] = base[n];
+    });
+}
+
+// ---------------------------------------------------------------------------------------
+// Boys function.  The table holds F_m(i * 0.1) for i = 0..300, m = 0..BOYS_ORDERS-1.
+// A kernel whose highest order is L copies the orders [first, first+nord) into LDS.
+// ---------------------------------------------------------------------------------------
+constexpr int BOYS_POINTS = 301;
+constexpr int BOYS_ORDERS = 17;
+
+template <class R>
+struct BoysTerms;
+template <>
+struct BoysTerms<double> {
+    static constexpr int nt = 8;  // |dT| <= 0.05: remainder < 1e-15 relative
+};
+template <>
+struct BoysTerms<float> {
+    static constexpr int nt = 5;  // remainder < 3e-9 relative
+};
+
+template <int L, class R>
+struct BoysSlice {
+    static constexpr int nt = BoysTerms<R>::nt;
+    static constexpr int first = (L <= 1) ? 0 : L;
+    static constexpr int nord = (L <= 1) ? L + nt : nt;
+};
+
+template <class R>
+__device__ __forceinline__ R grid_step();
+template <>
+__device__ __forceinline__ double grid_step<double>() { return 0.1; }
+template <>
+__device__ __forceinline__ float grid_step<float>() { return 0.1f; }
+
+template <class R>
+__device__ __forceinline__ R dev_sqrt(R x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float dev_sqrt<float>(float x) { return __fsqrt_rn(x); }
+template <class R>
+__device__ __forceinline__ R dev_exp(R x) { return exp(x); }
+template <>
+__device__ __forceinline__ float dev_exp<float>(float x) { return __expf(x); }
+
+// Copy the slice a kernel of total angular momentum L needs into LDS.
+template <int L, class R>
+__device__ __forceinline__ void load_boys(const R* __restrict__ g, R* lds) {
+    using S = BoysSlice<L, R>;
+    for (int x = threadIdx.x; x < BOYS_POINTS * S::nord; x += blockDim.x) {
+        const int i = x / S::nord, m = x - i * S::nord;
+        lds[x] = g[i * BOYS_ORDERS + S::first + m];
+    }
+}
+
+// Taylor sum of row[0..nt) at offset x = T0 - T: sum_k row[k] x^k / k!.
+template <int NT, class R>
+__device__ __forceinline__ R taylor(const R* row, R x) {
+    R t = row[NT - 1];
+    sfor<NT - 1>([&](auto K_) {
+        constexpr int k = NT - 2 - decltype(K_)::value;
+        t = row[k] + (x * R(1.0 / (k + 1))) * t;
+    });
+    return t;
+}
+
+// tab points at order S::first of grid point 0; consecutive grid points are STRIDE apart
+// (STRIDE = S::nord for an LDS slice, BOYS_ORDERS for the full table in global memory).
+template <int L, class R, int STRIDE = BoysSlice<L, R>::nord>
+__device__ __forceinline__ void boys(R T, const R* tab, R (&F)[L + 1]) {
+    using S = BoysSlice<L, R>;
+    if (T > R(30)) {
+        F[0] = R(0.8862269254527579) / dev_sqrt(T);
+        const R invT = R(1) / T;
+        sfor<L>([&](auto M_) {
+            constexpr int m = decltype(M_)::value;
+            F[m + 1] = (R(m) + R(0.5)) * F[m] * invT;
+        });
+        return;
+    }
+    int i = (int)(T * R(10) + R(0.5));
+    i = i < 0 ? 0 : (i > BOYS_POINTS - 1 ? BOYS_POINTS - 1 : i);  // NaN input stays in bounds
+    const R x = R(i) * grid_step<R>() - T;
+    const R* row = tab + i * STRIDE;
+    if constexpr (L <= 1) {
+        sfor<L + 1>([&](auto M_) {
+            constexpr int m = decltype(M_)::value;
+            F[m] = taylor<S::nt>(row + m, x);
+        });
+    } else {
+        F[L] = taylor<S::nt>(row, x);
+        const R e = dev_exp(-T);
+        const R twoT = T + T;
+        sfor<L>([&](auto M_) {
+            constexpr int m = L - decltype(M_)::value;  // m = L .. 1
+            F[m - 1] = (twoT * F[m] + e) * R(1.0 / (2 * m - 1));
+        });
+    }
+}
+
+}  // namespace md
+
+namespace md {
+template <class F, int... Is>
+inline void sfor_host_impl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(ic<Is>{}), ...);
+}
+// Host-side counterpart of sfor (launch loops over compile-time chunk indices).
+template <int N, class F>
+inline void sfor_host(F&& f) {
+    if constexpr (N > 0) sfor_host_impl(f, std::make_integer_sequence<int, N>{});
+}
+}  // namespace md

@@ -1,0 +1,82 @@
+"""Minimal HIP runtime access through ctypes: device buffers for device-resident J/K inputs.
+
+Only allocation, copies and synchronization; all compute goes through the engine library.
+"""
+import ctypes as C
+
+import numpy as np
+
+_hip = None
+
+
+def _lib():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipSetDevice.argtypes = [C.c_int]
+        _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+    return _hip
+
+
+H2D, D2H = 1, 2
+
+
+def device_count():
+    n = C.c_int(0)
+    if _lib().hipGetDeviceCount(C.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def synchronize():
+    rc = _lib().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError("hipDeviceSynchronize failed: %d" % rc)
+
+
+class DeviceArray:
+    """A float64 buffer in device memory."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = C.c_void_p()
+        rc = _lib().hipMalloc(C.byref(p), max(1, self.n) * 8)
+        if rc != 0:
+            raise MemoryError("hipMalloc(%d) failed: %d" % (self.n * 8, rc))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, order="F"))
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.n
+        rc = _lib().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.n * 8, H2D)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy H2D failed: %d" % rc)
+
+    def download(self):
+        out = np.empty(self.n)
+        rc = _lib().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.n * 8, D2H)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy D2H failed: %d" % rc)
+        return out
+
+    def free(self):
+        if self.ptr:
+            _lib().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

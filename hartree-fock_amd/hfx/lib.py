@@ -18,6 +18,7 @@ HFX_MIXED = 1
 HFX_JK_DEVICE_PTRS = 1
 HFX_JK_USE_GRAPH = 2
 HFX_JK_NO_STATS = 4
+HFX_JK_CLASS_TIMING = 8
 N_CLASSES = 21
 
 ERRORS = {1: "invalid argument", 2: "unsupported", 3: "out of memory", 4: "device error",
@@ -43,12 +44,16 @@ class HfxJkArgs(C.Structure):
 class HfxJkStats(C.Structure):
     _fields_ = [("n_quartets", C.c_int64), ("n_fp32", C.c_int64), ("class_count", C.c_int64 * N_CLASSES),
                 ("hash_sum", C.c_uint64), ("hash_xor", C.c_uint64), ("model_gflop", C.c_double),
-                ("device_ms", C.c_double)]
+                ("device_ms", C.c_double), ("class_gflop64", C.c_double * N_CLASSES),
+                ("class_gflop32", C.c_double * N_CLASSES), ("class_ms64", C.c_double * N_CLASSES),
+                ("class_ms32", C.c_double * N_CLASSES)]
 
     def as_dict(self):
         return dict(n_quartets=self.n_quartets, n_fp32=self.n_fp32, class_count=list(self.class_count),
                     hash_sum=self.hash_sum, hash_xor=self.hash_xor, model_gflop=self.model_gflop,
-                    device_ms=self.device_ms)
+                    device_ms=self.device_ms, class_gflop64=list(self.class_gflop64),
+                    class_gflop32=list(self.class_gflop32), class_ms64=list(self.class_ms64),
+                    class_ms32=list(self.class_ms32))
 
 
 class HfxScfOptions(C.Structure):
@@ -254,6 +259,20 @@ class Engine:
         st = HfxJkStats()
         self.lib.check(self.lib.dll.hfx_jk(self.h, C.byref(a), _p(J), _p(Ka), _p(Kb), C.byref(st)), "hfx_jk")
         return self.unpack(J), self.unpack(Ka), (None if Kb is None else self.unpack(Kb)), st.as_dict()
+
+
+def jk_device(engine, dD_tot, dD_alpha, J, Ka, dD_beta=None, Kb=None, Q=None, schwarz_thresh=1e-10,
+              density_thresh=1e-10, precision=HFX_FP64, fp32_cut=0.0, flags=0):
+    """One build on device-resident buffers (hfx.device.DeviceArray); returns the stats dict."""
+    a = HfxJkArgs()
+    vp = lambda d: None if d is None else C.cast(d.ptr, _f64p)
+    a.dD_tot, a.dD_alpha, a.dD_beta = vp(dD_tot), vp(dD_alpha), vp(dD_beta)
+    a.Q = None if Q is None else _p(Q)
+    a.schwarz_thresh, a.density_thresh = schwarz_thresh, density_thresh
+    a.precision, a.fp32_cut, a.flags = precision, fp32_cut, flags | HFX_JK_DEVICE_PTRS
+    st = HfxJkStats()
+    engine.lib.check(engine.lib.dll.hfx_jk(engine.h, C.byref(a), vp(J), vp(Ka), vp(Kb), C.byref(st)), "hfx_jk")
+    return st.as_dict()
 
 
 def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, outputs=True, trace_cap=200):

@@ -38,7 +38,8 @@ enum { HFX_FP64 = 0, HFX_MIXED = 1 };
 enum {
     HFX_JK_DEVICE_PTRS = 1, /* density inputs and J/K outputs are device pointers           */
     HFX_JK_USE_GRAPH = 2,   /* replay the engine's captured hipGraph of the build pipeline   */
-    HFX_JK_NO_STATS = 4     /* skip the survivor statistics read-back                         */
+    HFX_JK_NO_STATS = 4,    /* skip the survivor statistics read-back                         */
+    HFX_JK_CLASS_TIMING = 8 /* time every class kernel group with HIP events (stats.class_ms*) */
 };
 
 #define HFX_MAX_L 2
@@ -94,6 +95,10 @@ typedef struct hfx_jk_stats {
     uint64_t hash_xor;                    /* xor of the same                              */
     double model_gflop;                   /* algorithmic GFLOP of the surviving work      */
     double device_ms;                     /* device time of the build (HIP events)        */
+    double class_gflop64[HFX_N_CLASSES];  /* model GFLOP of each class's FP64 / FP32 part */
+    double class_gflop32[HFX_N_CLASSES];
+    double class_ms64[HFX_N_CLASSES];     /* HFX_JK_CLASS_TIMING: device ms of each part  */
+    double class_ms32[HFX_N_CLASSES];
 } hfx_jk_stats;
 
 /* IntegralEngine::IntegralEngine (src/IntegralEngine.cpp:13-75).  Builds the device basis,

@@ -636,6 +636,7 @@ void direct_jk(Mol& m, const double* dDt, const double* dDa, const double* dDb, 
     std::vector<double> Jt((size_t)N * N, 0.0), Kat((size_t)N * N, 0.0), Kbt;
     if (uhf) Kbt.assign((size_t)N * N, 0.0);
     int64_t cls[HFX_N_CLASSES] = {0};
+    double clsfl[HFX_N_CLASSES] = {0};
     uint64_t hsum_ = 0, hxor = 0;
     int64_t nsurv = 0;
     double mflop = 0;
@@ -649,6 +650,7 @@ void direct_jk(Mol& m, const double* dDt, const double* dDa, const double* dDb, 
         std::vector<double> Jp((size_t)N * N, 0.0), Kap((size_t)N * N, 0.0), Kbp;
         if (uhf) Kbp.assign((size_t)N * N, 0.0);
         int64_t lcls[HFX_N_CLASSES] = {0};
+        double lclsfl[HFX_N_CLASSES] = {0};
         uint64_t lhs = 0, lhx = 0;
         int64_t lns = 0;
         double lfl = 0;
@@ -667,13 +669,16 @@ void direct_jk(Mol& m, const double* dDt, const double* dDa, const double* dDb, 
                         G.assign((size_t)nA * nB * nC * nD, 0.0);
                         eri_quartet(m, a, b, c, d, G.data());
                         lns++;
-                        lcls[quartet_class(m.l[a], m.l[b], m.l[c], m.l[d])]++;
+                        const int qcls = quartet_class(m.l[a], m.l[b], m.l[c], m.l[d]);
+                        lcls[qcls]++;
                         const uint64_t h = splitmix64((((uint64_t)a * nS + b) * nS + c) * nS + d);
                         lhs += h;
                         lhx ^= h;
-                        lfl += (double)m.nprim[a] * m.nprim[b] * m.nprim[c] * m.nprim[d] *
-                                   model_flop(m.l[a], m.l[b], m.l[c], m.l[d]) +
-                               12.0 * nA * nB * nC * nD;
+                        const double qfl = (double)m.nprim[a] * m.nprim[b] * m.nprim[c] * m.nprim[d] *
+                                               model_flop(m.l[a], m.l[b], m.l[c], m.l[d]) +
+                                           12.0 * nA * nB * nC * nD;
+                        lfl += qfl;
+                        lclsfl[qcls] += qfl;
                         const size_t nCD = (size_t)nC * nD, nBCD = nB * nCD;
                         // Digestion (SCF.cpp:633-711).
                         for (int A = 0; A < nA; ++A) {
@@ -726,7 +731,10 @@ void direct_jk(Mol& m, const double* dDt, const double* dDa, const double* dDb, 
                 Kat[x] += Kap[x];
                 if (uhf) Kbt[x] += Kbp[x];
             }
-            for (int q = 0; q < HFX_N_CLASSES; ++q) cls[q] += lcls[q];
+            for (int q = 0; q < HFX_N_CLASSES; ++q) {
+                cls[q] += lcls[q];
+                clsfl[q] += lclsfl[q];
+            }
             hsum_ += lhs;
             hxor ^= lhx;
             nsurv += lns;
@@ -743,7 +751,10 @@ void direct_jk(Mol& m, const double* dDt, const double* dDa, const double* dDb, 
     if (uhf && Kb) sym(Kbt, Kb);
     if (st) {
         st->n_quartets += nsurv;
-        for (int q = 0; q < HFX_N_CLASSES; ++q) st->class_count[q] += cls[q];
+        for (int q = 0; q < HFX_N_CLASSES; ++q) {
+            st->class_count[q] += cls[q];
+            st->class_gflop64[q] += clsfl[q] * 1e-9;
+        }
         st->hash_sum += hsum_;
         st->hash_xor ^= hxor;
         st->model_gflop += mflop * 1e-9;
