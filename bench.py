@@ -150,8 +150,17 @@ def main():
     bmx = args.steps / tmx
     log("FP64 builds/s %.3f (%.2f ms, device %.2f ms, %.1f model GFLOP, %d quartets)" % (
         b64, 1e3 * t64 / args.steps, st64["device_ms"], st64["model_gflop"], st64["n_quartets"]))
+    for i in range(21):
+        if st64["class_count"][i]:
+            log("  %s: %10d quartets %8.1f GFLOP  fp64 %7.3f ms (%.2f TFLOP/s)" % (
+                CLASS_NAMES[i], st64["class_count"][i], st64["class_gflop64"][i], c64[i],
+                st64["class_gflop64"][i] / c64[i] if c64[i] > 0 else 0))
     log("mixed builds/s %.3f (%.2f ms, device %.2f ms, fp32 quartets %d)" % (
         bmx, 1e3 * tmx / args.steps, stmx["device_ms"], stmx["n_fp32"]))
+    for i in range(21):
+        if stmx["class_count"][i]:
+            log("  %s: fp64 %7.3f ms %8.1f GFLOP | fp32 %7.3f ms %8.1f GFLOP" % (
+                CLASS_NAMES[i], cm64[i], stmx["class_gflop64"][i], cm32[i], stmx["class_gflop32"][i]))
 
     # dominant kernel group of the headline (mixed) build
     groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
@@ -199,6 +208,7 @@ def main():
         "build_dE_mixed": dE_build,
         "build_max_dJK_mixed": dF_build,
         "class_ms_f64_f32": per_class,
+        "class_ms_fp64_build": {CLASS_NAMES[i]: round(c64[i], 3) for i in range(21) if c64[i] > 0},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

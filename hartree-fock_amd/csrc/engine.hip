@@ -178,9 +178,8 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
 
 struct ScreenArgs {
     // bra class list
-    const int* bra_ids;   // pair ids in sorted order
+    const int* bra_ids;  // pair ids in sorted order
     int n_bra;
-    int bra_pos0;         // position of bra_ids[0] in its class list (0)
     // ket class list
     const int* ket_ids;
     const double* ket_q;  // Q of each ket-list position
@@ -189,81 +188,130 @@ struct ScreenArgs {
     const int* ket_molseg;  // [mol] -> first segment, [mol+1] -> end (per ket class)
     bool diagonal;          // bra class == ket class: only ket positions <= bra position
     // per pair / shell data
-    const int* pair_a;
-    const int* pair_b;
-    const int* pair_np;
+    const int4* pair_info;  // {shell a, shell b, local a | local b << 16, nprim pairs}
     const int* pair_mol;
     const double* pair_q;
-    const int* sh_local;
     const long long* sh_qbase;
     const int* sh_ns;
     const double *Mt, *Ma, *Mb;
+    int ns_max;             // LDS row length (largest molecule's shell count)
     double tauS, tauD, cut;
     bool mixed;
     long long nS_total;
     // output
     uint2* list;
     long long reg_lo, reg_hi;
-    unsigned long long* cnt;  // [0] FP64 count, [1] FP32 count of this class
+    unsigned long long* cnt;    // [0] FP64 count, [1] FP32 count of this class
     unsigned long long* stats;  // [0] hash sum, [1] hash xor, [2]/[3] primitive quartets FP64/FP32
 };
 
-// One wavefront per bra pair.  Ket candidates: for every nprim segment of the bra's
+// One wavefront per bra pair (a, b).  Ket candidates: for every nprim segment of the bra's
 // molecule in the ket class, the Schwarz prefix (segments are sorted by Q descending, so
-// Q(ab)Q(cd) >= tau_S holds on a prefix), then the density test per candidate.
+// Q(ab)Q(cd) >= tau_S holds on a prefix), then the density test per candidate.  The rows a
+// and b of the spin-density maxima are staged in LDS (dynamic shared memory, one region per
+// wavefront).
 __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
+    extern __shared__ double srow[];
     const int lane = threadIdx.x & 63;
-    const int bpos = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (bpos >= a.n_bra) return;  // uniform per wavefront; no barriers below
-    const int pid = a.bra_ids[bpos];
-    const int sa = a.pair_a[pid], sb = a.pair_b[pid];
-    const int mol = a.pair_mol[pid];
-    const double qab = a.pair_q[pid];
-    const long long qb = a.sh_qbase[sa];
-    const long long ns = a.sh_ns[sa];
-    const int la = a.sh_local[sa], lb = a.sh_local[sb];
-    const double tab2 = 2.0 * a.Mt[qb + la + (long long)lb * ns];
-    const long long npa = a.pair_np[pid];
+    const int wib = threadIdx.x >> 6;
+    const int bpos = blockIdx.x * (blockDim.x >> 6) + wib;
+    const bool active = bpos < a.n_bra;
+    const int nrow = a.Mb ? 4 : 2;
+    double* rows = srow + (size_t)wib * nrow * a.ns_max;
+    int pid = 0, sa = 0, sb = 0, la = 0, lb = 0, mol = 0;
+    long long qb = 0, ns = 0;
+    double qab = 0.0, tab2 = 0.0;
+    long long npa = 0;
+    if (active) {
+        pid = a.bra_ids[bpos];
+        const int4 pi = a.pair_info[pid];
+        sa = pi.x;
+        sb = pi.y;
+        la = pi.z & 0xffff;
+        lb = pi.z >> 16;
+        npa = pi.w;
+        mol = a.pair_mol[pid];
+        qab = a.pair_q[pid];
+        qb = a.sh_qbase[sa];
+        ns = a.sh_ns[sa];
+        tab2 = 2.0 * a.Mt[qb + la + lb * ns];
+        for (int x = lane; x < ns; x += 64) {
+            rows[x] = a.Ma[qb + la + (long long)x * ns];
+            rows[a.ns_max + x] = a.Ma[qb + lb + (long long)x * ns];
+            if (a.Mb) {
+                rows[2 * a.ns_max + x] = a.Mb[qb + la + (long long)x * ns];
+                rows[3 * a.ns_max + x] = a.Mb[qb + lb + (long long)x * ns];
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;  // uniform per wavefront; no barriers below
+    const double* ra = rows;
+    const double* rb = rows + a.ns_max;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0;
     const int sg0 = a.ket_molseg[mol], sg1 = a.ket_molseg[mol + 1];
-    for (int sg = sg0; sg < sg1; ++sg) {
-        const int s0 = a.ket_seg_start[sg];
-        int s1 = a.ket_seg_end[sg];
-        if (a.diagonal) s1 = min(s1, a.bra_pos0 + bpos + 1);
-        if (s1 <= s0) continue;
-        // Schwarz prefix: first x in [s0, s1) with qab * Q(x) < tau_S.
-        int lo = s0, hi = s1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (qab * a.ket_q[mid] < a.tauS)
-                hi = mid;
-            else
-                lo = mid + 1;
+    // Pass 0 counts this bra's survivors, one reservation per list part follows, and pass 1
+    // writes them: the bra's entries are contiguous in the work list.
+    unsigned long long n64 = 0, n32 = 0, b64 = 0, b32 = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            if (lane == 0) {
+                if (n64) b64 = atomicAdd(&a.cnt[0], n64);
+                if (n32) b32 = atomicAdd(&a.cnt[1], n32);
+            }
+            b64 = __shfl(b64, 0);
+            b32 = __shfl(b32, 0);
         }
-        const int xend = lo;
-        for (int x0 = s0; x0 < xend; x0 += 64) {
-            const int x = x0 + lane;
-            bool keep = false, f32 = false;
-            int qid = 0;
-            if (x < xend) {
-                qid = a.ket_ids[x];
-                const int sc = a.pair_a[qid], sd = a.pair_b[qid];
-                const int lc = a.sh_local[sc], ld = a.sh_local[sd];
-                double mx = fmax(tab2, 2.0 * a.Mt[qb + lc + (long long)ld * ns]);
-                mx = fmax(mx, a.Ma[qb + la + (long long)lc * ns]);
-                mx = fmax(mx, a.Ma[qb + la + (long long)ld * ns]);
-                mx = fmax(mx, a.Ma[qb + lb + (long long)lc * ns]);
-                mx = fmax(mx, a.Ma[qb + lb + (long long)ld * ns]);
-                if (a.Mb) {
-                    mx = fmax(mx, a.Mb[qb + la + (long long)lc * ns]);
-                    mx = fmax(mx, a.Mb[qb + la + (long long)ld * ns]);
-                    mx = fmax(mx, a.Mb[qb + lb + (long long)lc * ns]);
-                    mx = fmax(mx, a.Mb[qb + lb + (long long)ld * ns]);
+        for (int sg = sg0; sg < sg1; ++sg) {
+            const int s0 = a.ket_seg_start[sg];
+            int s1 = a.ket_seg_end[sg];
+            if (a.diagonal) s1 = min(s1, bpos + 1);
+            if (s1 <= s0) continue;
+            // Schwarz prefix: first x in [s0, s1) with qab * Q(x) < tau_S.
+            int lo = s0, hi = s1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (qab * a.ket_q[mid] < a.tauS)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            const int xend = lo;
+            for (int x0 = s0; x0 < xend; x0 += 64) {
+                const int x = x0 + lane;
+                bool keep = false, f32 = false;
+                int qid = 0;
+                int4 pk = make_int4(0, 0, 0, 0);
+                double mx = 0.0;
+                if (x < xend) {
+                    qid = a.ket_ids[x];
+                    pk = a.pair_info[qid];
+                    const int lc = pk.z & 0xffff, ld = pk.z >> 16;
+                    mx = fmax(tab2, 2.0 * a.Mt[qb + lc + (long long)ld * ns]);
+                    mx = fmax(mx, fmax(fmax(ra[lc], ra[ld]), fmax(rb[lc], rb[ld])));
+                    if (a.Mb) {
+                        const double* rc = rows + 2 * a.ns_max;
+                        const double* rd = rows + 3 * a.ns_max;
+                        mx = fmax(mx, fmax(fmax(rc[lc], rc[ld]), fmax(rd[lc], rd[ld])));
+                    }
+                    keep = !(mx < a.tauD);
+                    f32 = keep && a.mixed && (qab * a.ket_q[x] * mx < a.cut);
                 }
-                keep = !(mx < a.tauD);
+                const unsigned long long m64 = __ballot(keep && !f32);
+                const unsigned long long m32 = __ballot(keep && f32);
+                if (pass == 0) {
+                    n64 += __popcll(m64);
+                    n32 += __popcll(m32);
+                    continue;
+                }
                 if (keep) {
-                    f32 = a.mixed && (qab * a.ket_q[x] * mx < a.cut);
+                    const uint2 ent = make_uint2((unsigned)pid, (unsigned)qid);
+                    if (!f32)
+                        a.list[a.reg_lo + (long long)(b64 + __popcll(m64 & lt_mask))] = ent;
+                    else
+                        a.list[a.reg_hi - 1 - (long long)(b32 + __popcll(m32 & lt_mask))] = ent;
+                    const int sc = pk.x, sd = pk.y;
                     // Canonical (reference loop order) form of the quartet for the hash.
                     long long A = max(sa, sb), B = min(sa, sb), C = max(sc, sd), D = min(sc, sd);
                     if (A * (A + 1) / 2 + B < C * (C + 1) / 2 + D) {
@@ -279,26 +327,12 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                     h_sum += h;
                     h_xor ^= h;
                     if (f32)
-                        prims32 += (unsigned long long)(npa * a.pair_np[qid]);
+                        prims32 += (unsigned long long)(npa * pk.w);
                     else
-                        prims += (unsigned long long)(npa * a.pair_np[qid]);
+                        prims += (unsigned long long)(npa * pk.w);
                 }
-            }
-            const unsigned long long m64 = __ballot(keep && !f32);
-            const unsigned long long m32 = __ballot(keep && f32);
-            unsigned long long b64 = 0, b32 = 0;
-            if (lane == 0) {
-                if (m64) b64 = atomicAdd(&a.cnt[0], (unsigned long long)__popcll(m64));
-                if (m32) b32 = atomicAdd(&a.cnt[1], (unsigned long long)__popcll(m32));
-            }
-            b64 = __shfl(b64, 0);
-            b32 = __shfl(b32, 0);
-            if (keep) {
-                const uint2 ent = make_uint2((unsigned)pid, (unsigned)qid);
-                if (!f32)
-                    a.list[a.reg_lo + (long long)(b64 + __popcll(m64 & lt_mask))] = ent;
-                else
-                    a.list[a.reg_hi - 1 - (long long)(b32 + __popcll(m32 & lt_mask))] = ent;
+                b64 += __popcll(m64);
+                b32 += __popcll(m32);
             }
         }
     }
@@ -326,15 +360,19 @@ __global__ void k_ranges(const unsigned long long* cnt, const long long* reg, lo
     ranges[2 * N_COMBOS + 2 * k + 1] = reg[k + 1];
 }
 
-// out = A + A^T within every molecule block (blockIdx.y = molecule).
-__global__ void k_symm(const double* A, double* out, const long long* mbase, const int* mn) {
+// out = sum over the accumulator copies of A + A^T, within every molecule block
+// (blockIdx.y = molecule).
+__global__ void k_symm(const double* A, long long stride, int copies, double* out, const long long* mbase,
+                       const int* mn) {
     const int m = blockIdx.y;
     const long long base = mbase[m];
     const long long N = mn[m];
     const long long n = N * N;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
         const long long r = e % N, c = e / N;
-        out[base + e] = A[base + r + c * N] + A[base + c + r * N];
+        double v = 0.0;
+        for (int k = 0; k < copies; ++k) v += A[k * stride + base + r + c * N] + A[k * stride + base + c + r * N];
+        out[base + e] = v;
     }
 }
 
@@ -543,6 +581,8 @@ struct hfx_engine {
 
     // device copies
     int *d_pair_a = nullptr, *d_pair_b = nullptr, *d_pair_ps = nullptr, *d_pair_np = nullptr, *d_pair_mol = nullptr;
+    int4* d_pair_info = nullptr;
+    int ns_max = 0;
     double* d_pair_q = nullptr;
     PrimPair64* d_pp64 = nullptr;
     PrimPair32* d_pp32 = nullptr;
@@ -562,7 +602,8 @@ struct hfx_engine {
     // per-build buffers
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Db = nullptr;
     double *d_Mt = nullptr, *d_Ma = nullptr, *d_Mb = nullptr;
-    double *d_AJ = nullptr, *d_AKa = nullptr, *d_AKb = nullptr;
+    double *d_AJ = nullptr, *d_AKa = nullptr, *d_AKb = nullptr;  // ACC_COPIES copies each
+    static constexpr int ACC_COPIES = 8;  // one per XCD of an MI355X
     double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
     uint2* d_list = nullptr;
     long long list_cap = 0;
@@ -691,6 +732,8 @@ EriArgs eri_args(hfx_engine* e) {
     a.sh_cf = e->d_sh_cf;
     a.boys64 = e->d_boys64;
     a.boys32 = e->d_boys32;
+    a.n_copies = 1;
+    a.acc_stride = 0;
     return a;
 }
 
@@ -887,6 +930,13 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
             }
     }
     e->n_pp = (long long)pp64.size();
+    std::vector<int4> pinfo(e->n_pairs);
+    for (int p = 0; p < e->n_pairs; ++p) {
+        const int a = e->pair_a[p], b = e->pair_b[p];
+        pinfo[p] = make_int4(a, b, e->sh_local[a] | (e->sh_local[b] << 16), e->pair_np[p]);
+    }
+    for (int m = 0; m < n_mol; ++m) e->ns_max = std::max(e->ns_max, e->mol_nS[m]);
+    if (e->ns_max > 65535) return fail(HFX_ERR_UNSUPPORTED, "more than 65535 shells in one molecule");
     std::vector<PrimPair32> pp32(pp64.size());
     for (size_t i = 0; i < pp64.size(); ++i) {
         const PrimPair64& s = pp64[i];
@@ -927,6 +977,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     if ((rc = upload(e.get(), e->d_pair_a, e->pair_a)) || (rc = upload(e.get(), e->d_pair_b, e->pair_b)) ||
         (rc = upload(e.get(), e->d_pair_ps, e->pair_ps)) || (rc = upload(e.get(), e->d_pair_np, e->pair_np)) ||
         (rc = upload(e.get(), e->d_pair_mol, e->pair_mol)) || (rc = upload(e.get(), e->d_pp64, pp64)) ||
+        (rc = upload(e.get(), e->d_pair_info, pinfo)) ||
         (rc = upload(e.get(), e->d_pp32, pp32)) || (rc = upload(e.get(), e->d_sh_ao, e->sh_ao)) ||
         (rc = upload(e.get(), e->d_sh_nao, e->sh_nao)) || (rc = upload(e.get(), e->d_sh_mn, e->sh_mn)) ||
         (rc = upload(e.get(), e->d_sh_local, e->sh_local)) || (rc = upload(e.get(), e->d_sh_ns, e->sh_ns)) ||
@@ -946,9 +997,9 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     e->d_Mt = e->mem.alloc<double>(nq);
     e->d_Ma = e->mem.alloc<double>(nq);
     e->d_Mb = e->mem.alloc<double>(nq);
-    e->d_AJ = e->mem.alloc<double>(ne);
-    e->d_AKa = e->mem.alloc<double>(ne);
-    e->d_AKb = e->mem.alloc<double>(ne);
+    e->d_AJ = e->mem.alloc<double>(ne * hfx_engine::ACC_COPIES);
+    e->d_AKa = e->mem.alloc<double>(ne * hfx_engine::ACC_COPIES);
+    e->d_AKb = e->mem.alloc<double>(ne * hfx_engine::ACC_COPIES);
     e->d_J = e->mem.alloc<double>(ne);
     e->d_Ka = e->mem.alloc<double>(ne);
     e->d_Kb = e->mem.alloc<double>(ne);
@@ -1034,24 +1085,21 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
         ScreenArgs a{};
         a.bra_ids = e->d_cls_ids[p1];
         a.n_bra = (int)e->cls_ids[p1].size();
-        a.bra_pos0 = 0;
         a.ket_ids = e->d_cls_ids[p2];
         a.ket_q = e->d_cls_q[p2];
         a.ket_seg_start = e->d_seg_start[p2];
         a.ket_seg_end = e->d_seg_end[p2];
         a.ket_molseg = e->d_molseg[p2];
         a.diagonal = (p1 == p2);
-        a.pair_a = e->d_pair_a;
-        a.pair_b = e->d_pair_b;
-        a.pair_np = e->d_pair_np;
+        a.pair_info = e->d_pair_info;
         a.pair_mol = e->d_pair_mol;
         a.pair_q = e->d_pair_q;
-        a.sh_local = e->d_sh_local;
         a.sh_qbase = e->d_sh_qbase;
         a.sh_ns = e->d_sh_ns;
         a.Mt = e->d_Mt;
         a.Ma = e->d_Ma;
         a.Mb = uhf ? e->d_Mb : nullptr;
+        a.ns_max = e->ns_max;
         a.tauS = args->schwarz_thresh;
         a.tauD = args->density_thresh;
         a.cut = args->fp32_cut;
@@ -1062,15 +1110,18 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
         a.reg_hi = e->reg[k + 1];
         a.cnt = e->d_cnt + 2 * k;
         a.stats = e->d_stats + 4 * k;
-        const int waves = a.n_bra;
-        hipLaunchKernelGGL(k_screen, dim3((waves + 3) / 4), dim3(256), 0, s, a);
+        const size_t row_bytes = (size_t)(uhf ? 4 : 2) * e->ns_max * sizeof(double);
+        const int wpb = (int)std::min<size_t>(4, (60 * 1024) / row_bytes);
+        if (wpb < 1) return fail(HFX_ERR_UNSUPPORTED, "molecule too large for the screening kernel's LDS rows");
+        hipLaunchKernelGGL(k_screen, dim3((a.n_bra + wpb - 1) / wpb), dim3(64 * wpb), wpb * row_bytes, s, a);
     }
     hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_cnt, e->d_reg, e->d_ranges);
     {
         const int g = e->n_cu * 4;
-        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, ne, 0.0);
-        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, ne, 0.0);
-        if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, ne, 0.0);
+        const long long nc = ne * hfx_engine::ACC_COPIES;
+        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, nc, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, nc, 0.0);
+        if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, nc, 0.0);
     }
     EriArgs ea = eri_args(e);
     ea.list = e->d_list;
@@ -1080,6 +1131,8 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     ea.AJ = e->d_AJ;
     ea.AKa = e->d_AKa;
     ea.AKb = e->d_AKb;
+    ea.acc_stride = ne;
+    ea.n_copies = hfx_engine::ACC_COPIES;
     ea.tauS = args->schwarz_thresh;
     ea.out = nullptr;
     ea.out_off = nullptr;
@@ -1109,9 +1162,10 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     }
     {
         const dim3 g(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256))), e->n_mol);
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, e->d_J, e->d_mbase, e->d_mn);
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, e->d_Ka, e->d_mbase, e->d_mn);
-        if (uhf) hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, e->d_Kb, e->d_mbase, e->d_mn);
+        const int cp = hfx_engine::ACC_COPIES;
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, ne, cp, e->d_J, e->d_mbase, e->d_mn);
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, ne, cp, e->d_Ka, e->d_mbase, e->d_mn);
+        if (uhf) hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, ne, cp, e->d_Kb, e->d_mbase, e->d_mn);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev1, s));

@@ -61,12 +61,14 @@ struct EriArgs {
     const double* Dt;
     const double* Da;
     const double* Db;  // NULL: no K_beta
-    double* AJ;
-    double* AKa;
+    double* AJ;    // accumulators: n_copies copies, acc_stride elements apart; each
+    double* AKa;   // XCD adds into its own copy (selected by the XCC_ID hardware register)
     double* AKb;
+    long long acc_stride;
+    int n_copies;
     double tauS;
-    double* out;               // MODE_STORE: blocks [ab][cd], canonical class order
-    const long long* out_off;  // MODE_STORE: offset of each entry's block
+    double* out;               // store mode (non-NULL): blocks [ab][cd], canonical class order
+    const long long* out_off;  // store mode: offset of each entry's block
     const double* boys64;
     const float* boys32;
 };

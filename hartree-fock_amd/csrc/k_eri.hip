@@ -61,9 +61,13 @@ __device__ __forceinline__ const double* boys_of<double>(const EriArgs& a) { ret
 template <>
 __device__ __forceinline__ const float* boys_of<float>(const EriArgs& a) { return a.boys32; }
 
+// Accumulation into the copy private to this XCD: no other XCD touches these lines.
 __device__ __forceinline__ void atomic_add(double* p, double v) {
-    if (v != 0.0) unsafeAtomicAdd(p, v);
+    if (v != 0.0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Index of the XCD running this workgroup (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcd_id() { return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u); }
 
 // Is bra component row ia (of shell a) / ib (of shell b) touched by chunk C0?
 __host__ __device__ constexpr bool chunk_has_a(int c0, int ia) {
@@ -78,26 +82,33 @@ __host__ __device__ constexpr bool chunk_has_b(int c0, int ib) {
 }
 
 template <class R, int C0>
-__device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R* tab) {
+__device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab) {
     const uint2 e = a.list[qi];
     const int pid = (int)e.x, qid = (int)e.y;
-    const int ps = a.pair_ps[pid], np = a.pair_np[pid];
-    const int qs = a.pair_ps[qid], nq = a.pair_np[qid];
+    const int ps = a.pair_ps[pid], np = valid ? a.pair_np[pid] : 0;
+    const int qs = a.pair_ps[qid], nq = valid ? a.pair_np[qid] : 0;
     const PrimPair<R>* __restrict__ pp = pp_of<R>(a);
 
     R acc[CB][NCD];
     sfor<CB>([&](auto c) { sfor<NCD>([&](auto d) { acc[c][d] = R(0); }); });
 
-    for (int ip = 0; ip < np; ++ip) {
-        const PrimPair<R> A = pp[ps + ip];
-        Ecoef<LA, LB, R> Ex, Ey, Ez;
-        hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
-        hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
-        hermite_e<LA, LB>(A.oo2p, A.paz, A.pbz, Ez);
-        for (int iq = 0; iq < nq; ++iq) {
-            const PrimPair<R> B = pp[qs + iq];
+    // Outer loop over the ket's primitive pairs (private to the lane), inner loop over the
+    // bra's: the work list groups quartets by bra pair, so the inner loads are shared by the
+    // lanes of a wavefront.
+    for (int iq = 0; iq < nq; ++iq) {
+        const PrimPair<R> B = pp[qs + iq];
+        Ecoef<LC, LD, R> Fx, Fy, Fz;
+        hermite_e<LC, LD>(B.oo2p, B.pax, B.pbx, Fx);
+        hermite_e<LC, LD>(B.oo2p, B.pay, B.pby, Fy);
+        hermite_e<LC, LD>(B.oo2p, B.paz, B.pbz, Fz);
+        for (int ip = 0; ip < np; ++ip) {
+            const PrimPair<R> A = pp[ps + ip];
+            Ecoef<LA, LB, R> Ex, Ey, Ez;
+            hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
+            hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
+            hermite_e<LA, LB>(A.oo2p, A.paz, A.pbz, Ez);
             const R pq = A.p + B.p;
-            const R rs = R(1) / dev_sqrt(pq);
+            const R rs = dev_rsqrt(pq);
             const R alpha = (A.p * B.p) * (rs * rs);
             const R X = A.px - B.px, Y = A.py - B.py, Z = A.pz - B.pz;
             const R T = alpha * (X * X + Y * Y + Z * Z);
@@ -115,11 +126,6 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R*
             }
             R Rt[nherm(LTOT)];
             hermite_r<LTOT>(base, X, Y, Z, Rt);
-
-            Ecoef<LC, LD, R> Fx, Fy, Fz;
-            hermite_e<LC, LD>(B.oo2p, B.pax, B.pbx, Fx);
-            hermite_e<LC, LD>(B.oo2p, B.pay, B.pby, Fy);
-            hermite_e<LC, LD>(B.oo2p, B.paz, B.pbz, Fz);
 
             sfor<CB>([&](auto c_) {
                 constexpr int c = decltype(c_)::value;
@@ -173,6 +179,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R*
     sfor<ND>([&](auto x) { cfd[x] = a.sh_cf[6 * sd + x]; });
 
     if (a.out) {  // store mode: normalized block, no screening
+        if (!valid) return;
         double* out = a.out + a.out_off[qi];
         sfor<CB>([&](auto c_) {
             constexpr int ab = C0 + decltype(c_)::value;
@@ -205,7 +212,8 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R*
             const double v = (double)acc[c][cd] * (cfa[ab / NB] * cfb[ab % NB]) * (cfc[cd / ND] * cfd[cd % ND]);
             return (fabs(v) < tauS) ? 0.0 : v * s;
         };
-        double* AJ = a.AJ + base;
+        const long long copy = (long long)(xcd_id() % a.n_copies) * a.acc_stride;
+        double* AJ = a.AJ + copy + base;
         {
             double Jab[CB], Jcd[NCD];
             sfor<CB>([&](auto x) { Jab[x] = 0.0; });
@@ -225,10 +233,33 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R*
                     }
                 });
             });
+            // Lanes of one wavefront that share the bra pair (contiguous runs of the work
+            // list) sum their J[a,b] block first; the first lane of each run flushes it.
+            const int lane = threadIdx.x & 63;
+            const int key = valid ? pid : -1 - lane;
+            const int prev = __shfl_up(key, 1);
+            const bool head = (lane == 0) || (prev != key);
+            const unsigned long long heads = __ballot(head);
+            const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+            const int run = __popcll(heads & le);
+            bool same[6];
+            sfor<6>([&](auto k_) {
+                constexpr int off = 1 << decltype(k_)::value;
+                const int r = __shfl_down(run, off);
+                same[decltype(k_)::value] = (lane + off < 64) && (r == run);
+            });
             sfor<CB>([&](auto c_) {
                 constexpr int c = decltype(c_)::value;
                 constexpr int ab = C0 + c;
-                if constexpr (ab < NAB) atomic_add(&AJ[(oA + ab / NB) + (oB + ab % NB) * N], 2.0 * Jab[c]);
+                if constexpr (ab < NAB) {
+                    double x = Jab[c];
+                    sfor<6>([&](auto k_) {
+                        constexpr int off = 1 << decltype(k_)::value;
+                        const double o = __shfl_down(x, off);
+                        if (same[decltype(k_)::value]) x += o;
+                    });
+                    if (head && valid) atomic_add(&AJ[(oA + ab / NB) + (oB + ab % NB) * N], 2.0 * x);
+                }
             });
             sfor<NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
@@ -291,8 +322,8 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, const R*
                 }
             });
         };
-        exchange(Da, a.AKa + base);
-        if (Db) exchange(Db, a.AKb + base);
+        exchange(Da, a.AKa + copy + base);
+        if (Db) exchange(Db, a.AKb + copy + base);
     }
 }
 
@@ -307,7 +338,12 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     load_boys<LTOT, R>(boys_of<R>(a), tab);
     __syncthreads();
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long w = first + threadIdx.x; w < n; w += stride) quartet<R, C0>(a, lo + w, tab);
+    // Block-uniform trip count: every lane reaches the wavefront reductions of the digest.
+    for (long long base = first; base < n; base += stride) {
+        const long long w = base + threadIdx.x;
+        const bool valid = w < n;
+        quartet<R, C0>(a, lo + (valid ? w : 0), valid, tab);
+    }
 }
 
 #ifndef HFX_CHUNK_BEGIN

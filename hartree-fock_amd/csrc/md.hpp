@@ -196,10 +196,20 @@ __device__ __forceinline__ double grid_step<double>() { return 0.1; }
 template <>
 __device__ __forceinline__ float grid_step<float>() { return 0.1f; }
 
+// 1/sqrt(x): the hardware estimate, refined by two Newton steps in FP64 (relative error of
+// a few ulp); the single-precision estimate is used as is in FP32.
 template <class R>
-__device__ __forceinline__ R dev_sqrt(R x) { return sqrt(x); }
+__device__ __forceinline__ R dev_rsqrt(R x);
 template <>
-__device__ __forceinline__ float dev_sqrt<float>(float x) { return __fsqrt_rn(x); }
+__device__ __forceinline__ double dev_rsqrt<double>(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * r, r, 1.0);
+    r = fma(0.5 * r, e, r);
+    e = fma(-x * r, r, 1.0);
+    return fma(0.5 * r, e, r);
+}
+template <>
+__device__ __forceinline__ float dev_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <class R>
 __device__ __forceinline__ R dev_exp(R x) { return exp(x); }
 template <>
@@ -232,8 +242,9 @@ template <int L, class R, int STRIDE = BoysSlice<L, R>::nord>
 __device__ __forceinline__ void boys(R T, const R* tab, R (&F)[L + 1]) {
     using S = BoysSlice<L, R>;
     if (T > R(30)) {
-        F[0] = R(0.8862269254527579) / dev_sqrt(T);
-        const R invT = R(1) / T;
+        const R rt = dev_rsqrt(T);
+        F[0] = R(0.8862269254527579) * rt;
+        const R invT = rt * rt;
         sfor<L>([&](auto M_) {
             constexpr int m = decltype(M_)::value;
             F[m + 1] = (R(m) + R(0.5)) * F[m] * invT;
