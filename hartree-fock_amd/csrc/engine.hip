@@ -9,10 +9,11 @@
 //
 // One J/K build (src/SCF.cpp:523-744) on the engine's stream:
 //   k_dmax        shell-pair maxima of |dD| (SCF.cpp:543-575)
-//   k_screen      per class: canonical quartets passing Q(a,b)Q(c,d) >= tau_S and the
-//                 density test (SCF.cpp:602-623), appended to the class's work list
-//                 (FP64 part from the region start, FP32 part from the region end)
-//   k_ranges      list ranges from the append counters
+//   k_screen<0>   per class: count, for every bra pair, the canonical quartets passing
+//                 Q(a,b)Q(c,d) >= tau_S and the density test (SCF.cpp:602-623)
+//   scan          exclusive prefix sum of the counts (hipcub)
+//   k_screen<1>   write the survivors at their scanned positions (FP64 and FP32 parts)
+//   k_ranges      list ranges from the scanned offsets
 //   k_eri<...>    contracted ERIs + digestion into AJ/AK (k_eri.hip)
 //   k_symm        J = AJ + AJ^T, K = AK + AK^T (the reference's selfadjointView, SCF.cpp:727-728)
 // No host synchronization inside the pipeline: kernels read list sizes from device memory.
@@ -31,6 +32,8 @@
 #include <numeric>
 #include <string>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "engine.hpp"
 #include "md.hpp"
@@ -200,8 +203,9 @@ struct ScreenArgs {
     long long nS_total;
     // output
     uint2* list;
-    long long reg_lo, reg_hi;
-    unsigned long long* cnt;    // [0] FP64 count, [1] FP32 count of this class
+    unsigned long long* counts;        // pass 0: survivors of bra i at [slot64 + i] / [slot32 + i]
+    const unsigned long long* offs;    // pass 1: exclusive scan of counts = list positions
+    long long slot64, slot32;
     unsigned long long* stats;  // [0] hash sum, [1] hash xor, [2]/[3] primitive quartets FP64/FP32
 };
 
@@ -209,7 +213,10 @@ struct ScreenArgs {
 // molecule in the ket class, the Schwarz prefix (segments are sorted by Q descending, so
 // Q(ab)Q(cd) >= tau_S holds on a prefix), then the density test per candidate.  The rows a
 // and b of the spin-density maxima are staged in LDS (dynamic shared memory, one region per
-// wavefront).
+// wavefront).  PASS 0 counts each bra's survivors; after an exclusive scan of the counts,
+// PASS 1 writes them at their positions, so every class list is ordered like its bra list
+// (primitive-pair count descending: the heaviest work first).
+template <int PASS>
 __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     extern __shared__ double srow[];
     const int lane = threadIdx.x & 63;
@@ -251,18 +258,12 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0;
     const int sg0 = a.ket_molseg[mol], sg1 = a.ket_molseg[mol + 1];
-    // Pass 0 counts this bra's survivors, one reservation per list part follows, and pass 1
-    // writes them: the bra's entries are contiguous in the work list.
     unsigned long long n64 = 0, n32 = 0, b64 = 0, b32 = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-            if (lane == 0) {
-                if (n64) b64 = atomicAdd(&a.cnt[0], n64);
-                if (n32) b32 = atomicAdd(&a.cnt[1], n32);
-            }
-            b64 = __shfl(b64, 0);
-            b32 = __shfl(b32, 0);
-        }
+    if (PASS == 1) {
+        b64 = a.offs[a.slot64 + bpos];
+        b32 = a.offs[a.slot32 + bpos];
+    }
+    {
         for (int sg = sg0; sg < sg1; ++sg) {
             const int s0 = a.ket_seg_start[sg];
             int s1 = a.ket_seg_end[sg];
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                 }
                 const unsigned long long m64 = __ballot(keep && !f32);
                 const unsigned long long m32 = __ballot(keep && f32);
-                if (pass == 0) {
+                if (PASS == 0) {
                     n64 += __popcll(m64);
                     n32 += __popcll(m32);
                     continue;
@@ -308,9 +309,9 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                 if (keep) {
                     const uint2 ent = make_uint2((unsigned)pid, (unsigned)qid);
                     if (!f32)
-                        a.list[a.reg_lo + (long long)(b64 + __popcll(m64 & lt_mask))] = ent;
+                        a.list[(long long)(b64 + __popcll(m64 & lt_mask))] = ent;
                     else
-                        a.list[a.reg_hi - 1 - (long long)(b32 + __popcll(m32 & lt_mask))] = ent;
+                        a.list[(long long)(b32 + __popcll(m32 & lt_mask))] = ent;
                     const int sc = pk.x, sd = pk.y;
                     // Canonical (reference loop order) form of the quartet for the hash.
                     long long A = max(sa, sb), B = min(sa, sb), C = max(sc, sd), D = min(sc, sd);
@@ -336,6 +337,13 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
             }
         }
     }
+    if (PASS == 0) {
+        if (lane == 0) {
+            a.counts[a.slot64 + bpos] = n64;
+            a.counts[a.slot32 + bpos] = n32;
+        }
+        return;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         h_sum += __shfl_xor(h_sum, off);
         h_xor ^= __shfl_xor(h_xor, off);
@@ -350,14 +358,15 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     }
 }
 
-// ranges[2k..2k+1] = FP64 entries of class k, ranges[42+2k..] = FP32 entries.
-__global__ void k_ranges(const unsigned long long* cnt, const long long* reg, long long* ranges) {
+// ranges[2k..2k+1] = FP64 entries of class k, ranges[42+2k..] = FP32 entries; slots[3k..3k+2]
+// are the class's first FP64 slot, first FP32 slot and end slot in the scanned counts.
+__global__ void k_ranges(const unsigned long long* offs, const long long* slots, long long* ranges) {
     const int k = threadIdx.x;
     if (k >= N_COMBOS) return;
-    ranges[2 * k] = reg[k];
-    ranges[2 * k + 1] = reg[k] + (long long)cnt[2 * k];
-    ranges[2 * N_COMBOS + 2 * k] = reg[k + 1] - (long long)cnt[2 * k + 1];
-    ranges[2 * N_COMBOS + 2 * k + 1] = reg[k + 1];
+    ranges[2 * k] = (long long)offs[slots[3 * k]];
+    ranges[2 * k + 1] = (long long)offs[slots[3 * k + 1]];
+    ranges[2 * N_COMBOS + 2 * k] = (long long)offs[slots[3 * k + 1]];
+    ranges[2 * N_COMBOS + 2 * k + 1] = (long long)offs[slots[3 * k + 2]];
 }
 
 // out = sum over the accumulator copies of A + A^T, within every molecule block
@@ -567,6 +576,7 @@ struct hfx_engine {
     // pairs
     int n_pairs = 0;
     std::vector<int> pair_a, pair_b, pair_ps, pair_np, pair_pc, pair_mol;
+    std::vector<int> pair_nprim;  // unpruned primitive-pair count (the model's work)
     std::map<long long, int> pair_of;
     long long n_pp = 0;
 
@@ -581,6 +591,9 @@ struct hfx_engine {
 
     // device copies
     int *d_pair_a = nullptr, *d_pair_b = nullptr, *d_pair_ps = nullptr, *d_pair_np = nullptr, *d_pair_mol = nullptr;
+    int *d_pair_ps32 = nullptr, *d_pair_np32 = nullptr;
+    std::vector<int> pair_ps32, pair_np32;
+    long long n_pp32 = 0;
     int4* d_pair_info = nullptr;
     int ns_max = 0;
     double* d_pair_q = nullptr;
@@ -607,7 +620,13 @@ struct hfx_engine {
     double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
     uint2* d_list = nullptr;
     long long list_cap = 0;
-    unsigned long long* d_cnt = nullptr;    // [2*N_COMBOS]
+    unsigned long long* d_counts = nullptr;  // [n_slots + 1] survivors per (class, precision, bra)
+    unsigned long long* d_offs = nullptr;    // exclusive scan of d_counts
+    long long n_slots = 0;
+    std::vector<long long> slots;            // [3 * N_COMBOS]: FP64 slot, FP32 slot, end slot
+    long long* d_slots = nullptr;
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
     unsigned long long* d_stats = nullptr;  // [4 * N_COMBOS]
     hipEvent_t cev[2 * N_COMBOS + 1] = {};
     long long* d_ranges = nullptr;          // [4*N_COMBOS]
@@ -724,6 +743,8 @@ EriArgs eri_args(hfx_engine* e) {
     a.pair_b = e->d_pair_b;
     a.pair_ps = e->d_pair_ps;
     a.pair_np = e->d_pair_np;
+    a.pair_ps32 = e->d_pair_ps32;
+    a.pair_np32 = e->d_pair_np32;
     a.pp64 = e->d_pp64;
     a.pp32 = e->d_pp32;
     a.sh_ao = e->d_sh_ao;
@@ -883,7 +904,9 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     }
 
     // ---- shell pairs and primitive-pair data ----
+    constexpr double PRUNE64 = 1e-20, PRUNE32 = 1e-11;
     std::vector<PrimPair64> pp64;
+    std::vector<PrimPair32> pp32;
     std::vector<int> cls_count(N_PAIR_CLASSES, 0);
     for (int m = 0; m < n_mol; ++m) {
         const int s0 = e->mol_shell0[m], s1 = s0 + e->mol_nS[m];
@@ -897,14 +920,15 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
                 e->pair_b.push_back(b);
                 e->pair_pc.push_back(pc);
                 e->pair_mol.push_back(m);
-                e->pair_ps.push_back((int)pp64.size());
-                e->pair_np.push_back(e->sh_np[a] * e->sh_np[b]);
+                e->pair_nprim.push_back(e->sh_np[a] * e->sh_np[b]);
                 e->pair_of[(long long)j * e->nS + i] = pid;
                 e->cls_ids[pc].push_back(pid);
                 cls_count[pc]++;
                 const double* A = &e->sh_ctr[3 * a];
                 const double* B = &e->sh_ctr[3 * b];
                 const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+                e->pair_ps.push_back((int)pp64.size());
+                e->pair_ps32.push_back((int)pp32.size());
                 for (int pa = 0; pa < e->sh_np[a]; ++pa)
                     for (int pb = 0; pb < e->sh_np[b]; ++pb) {
                         const double al = e->alpha[e->sh_po[a] + pa], be = e->alpha[e->sh_po[b] + pb];
@@ -924,26 +948,27 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
                         const double K = std::exp(-al * be * oop * AB2);
                         const double ga = e->norm[e->sh_no[a] + pa * e->sh_nao[a]];
                         const double gb = e->norm[e->sh_no[b] + pb * e->sh_nao[b]];
-                        d.kp = K * (e->coeff[e->sh_po[a] + pa] * ga) * (e->coeff[e->sh_po[b] + pb] * gb) * oop;
-                        pp64.push_back(d);
+                        const double kc = K * (e->coeff[e->sh_po[a] + pa] * ga) * (e->coeff[e->sh_po[b] + pb] * gb);
+                        d.kp = kc * oop;
+                        if (std::fabs(kc) >= PRUNE64) pp64.push_back(d);
+                        if (std::fabs(kc) >= PRUNE32)
+                            pp32.push_back(PrimPair32{(float)d.p, (float)d.px, (float)d.py, (float)d.pz, (float)d.kp,
+                                                      (float)d.pax, (float)d.pay, (float)d.paz, (float)d.pbx,
+                                                      (float)d.pby, (float)d.pbz, (float)d.oo2p});
                     }
+                e->pair_np.push_back((int)pp64.size() - e->pair_ps.back());
+                e->pair_np32.push_back((int)pp32.size() - e->pair_ps32.back());
             }
     }
     e->n_pp = (long long)pp64.size();
+    e->n_pp32 = (long long)pp32.size();
     std::vector<int4> pinfo(e->n_pairs);
     for (int p = 0; p < e->n_pairs; ++p) {
         const int a = e->pair_a[p], b = e->pair_b[p];
-        pinfo[p] = make_int4(a, b, e->sh_local[a] | (e->sh_local[b] << 16), e->pair_np[p]);
+        pinfo[p] = make_int4(a, b, e->sh_local[a] | (e->sh_local[b] << 16), e->pair_nprim[p]);
     }
     for (int m = 0; m < n_mol; ++m) e->ns_max = std::max(e->ns_max, e->mol_nS[m]);
     if (e->ns_max > 65535) return fail(HFX_ERR_UNSUPPORTED, "more than 65535 shells in one molecule");
-    std::vector<PrimPair32> pp32(pp64.size());
-    for (size_t i = 0; i < pp64.size(); ++i) {
-        const PrimPair64& s = pp64[i];
-        pp32[i] = PrimPair32{(float)s.p, (float)s.px, (float)s.py, (float)s.pz, (float)s.kp, (float)s.pax,
-                             (float)s.pay, (float)s.paz, (float)s.pbx, (float)s.pby, (float)s.pbz, (float)s.oo2p};
-    }
-
     // ---- Boys tables ----
     std::vector<double> b64((size_t)md::BOYS_POINTS * md::BOYS_ORDERS);
     std::vector<float> b32((size_t)md::BOYS_POINTS * md::BOYS_ORDERS);
@@ -977,7 +1002,8 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     if ((rc = upload(e.get(), e->d_pair_a, e->pair_a)) || (rc = upload(e.get(), e->d_pair_b, e->pair_b)) ||
         (rc = upload(e.get(), e->d_pair_ps, e->pair_ps)) || (rc = upload(e.get(), e->d_pair_np, e->pair_np)) ||
         (rc = upload(e.get(), e->d_pair_mol, e->pair_mol)) || (rc = upload(e.get(), e->d_pp64, pp64)) ||
-        (rc = upload(e.get(), e->d_pair_info, pinfo)) ||
+        (rc = upload(e.get(), e->d_pair_info, pinfo)) || (rc = upload(e.get(), e->d_pair_ps32, e->pair_ps32)) ||
+        (rc = upload(e.get(), e->d_pair_np32, e->pair_np32)) ||
         (rc = upload(e.get(), e->d_pp32, pp32)) || (rc = upload(e.get(), e->d_sh_ao, e->sh_ao)) ||
         (rc = upload(e.get(), e->d_sh_nao, e->sh_nao)) || (rc = upload(e.get(), e->d_sh_mn, e->sh_mn)) ||
         (rc = upload(e.get(), e->d_sh_local, e->sh_local)) || (rc = upload(e.get(), e->d_sh_ns, e->sh_ns)) ||
@@ -1004,11 +1030,26 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     e->d_Ka = e->mem.alloc<double>(ne);
     e->d_Kb = e->mem.alloc<double>(ne);
     e->d_list = e->mem.alloc<uint2>(e->list_cap);
-    e->d_cnt = e->mem.alloc<unsigned long long>(2 * N_COMBOS);
+    e->slots.assign(3 * N_COMBOS, 0);
+    for (int k = 0; k < N_COMBOS; ++k) {
+        const long long nb = (long long)e->cls_ids[combo_pc1(k)].size();
+        const bool live = !e->cls_ids[combo_pc1(k)].empty() && !e->cls_ids[combo_pc2(k)].empty();
+        e->slots[3 * k] = e->n_slots;
+        e->slots[3 * k + 1] = e->n_slots + (live ? nb : 0);
+        e->slots[3 * k + 2] = e->n_slots + (live ? 2 * nb : 0);
+        e->n_slots = e->slots[3 * k + 2];
+    }
+    e->d_counts = e->mem.alloc<unsigned long long>(e->n_slots + 1);
+    e->d_offs = e->mem.alloc<unsigned long long>(e->n_slots + 1);
+    if ((rc = upload(e.get(), e->d_slots, e->slots))) return rc;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, e->scan_tmp_bytes, e->d_counts, e->d_offs,
+                                            (int)(e->n_slots + 1), e->stream));
+    e->d_scan_tmp = e->mem.alloc<char>(e->scan_tmp_bytes);
     e->d_stats = e->mem.alloc<unsigned long long>(4 * N_COMBOS);
     e->d_ranges = e->mem.alloc<long long>(4 * N_COMBOS);
     if (!e->d_Dt || !e->d_Da || !e->d_Db || !e->d_Mt || !e->d_Ma || !e->d_Mb || !e->d_AJ || !e->d_AKa || !e->d_AKb ||
-        !e->d_J || !e->d_Ka || !e->d_Kb || !e->d_list || !e->d_cnt || !e->d_stats || !e->d_ranges)
+        !e->d_J || !e->d_Ka || !e->d_Kb || !e->d_list || !e->d_counts || !e->d_offs ||
+        !e->d_scan_tmp || !e->d_stats || !e->d_ranges)
         return fail(HFX_ERR_NOMEM, "hipMalloc failed for the build buffers");
 
     // ---- Schwarz table, then the class order it implies ----
@@ -1076,9 +1117,14 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
         d.n_pairs = e->n_pairs;
         hipLaunchKernelGGL(k_dmax, dim3((e->n_pairs + 255) / 256), dim3(256), 0, s, d);
     }
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_cnt, 2 * N_COMBOS);
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::max<long long>(1, std::min<long long>(1024, (e->n_slots + 256) / 256))),
+                       dim3(256), 0, s, e->d_counts, (int)(e->n_slots + 1));
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 4 * N_COMBOS);
     const bool mixed = args->precision == HFX_MIXED && args->fp32_cut > 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1)
+            HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
+                                                    (int)(e->n_slots + 1), s));
     for (int k = 0; k < N_COMBOS; ++k) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
@@ -1106,16 +1152,22 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
         a.mixed = mixed;
         a.nS_total = e->nS;
         a.list = e->d_list;
-        a.reg_lo = e->reg[k];
-        a.reg_hi = e->reg[k + 1];
-        a.cnt = e->d_cnt + 2 * k;
+        a.counts = e->d_counts;
+        a.offs = e->d_offs;
+        a.slot64 = e->slots[3 * k];
+        a.slot32 = e->slots[3 * k + 1];
         a.stats = e->d_stats + 4 * k;
         const size_t row_bytes = (size_t)(uhf ? 4 : 2) * e->ns_max * sizeof(double);
         const int wpb = (int)std::min<size_t>(4, (60 * 1024) / row_bytes);
         if (wpb < 1) return fail(HFX_ERR_UNSUPPORTED, "molecule too large for the screening kernel's LDS rows");
-        hipLaunchKernelGGL(k_screen, dim3((a.n_bra + wpb - 1) / wpb), dim3(64 * wpb), wpb * row_bytes, s, a);
+        const dim3 sg((a.n_bra + wpb - 1) / wpb), sb(64 * wpb);
+        if (pass == 0)
+            hipLaunchKernelGGL(k_screen<0>, sg, sb, wpb * row_bytes, s, a);
+        else
+            hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
     }
-    hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_cnt, e->d_reg, e->d_ranges);
+    }
+    hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_offs, e->d_slots, e->d_ranges);
     {
         const int g = e->n_cu * 4;
         const long long nc = ne * hfx_engine::ACC_COPIES;
@@ -1172,10 +1224,11 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     HIPCHK(hipMemcpyAsync(J, e->d_J, ne * sizeof(double), d2h, s));
     HIPCHK(hipMemcpyAsync(Ka, e->d_Ka, ne * sizeof(double), d2h, s));
     if (uhf) HIPCHK(hipMemcpyAsync(Kb, e->d_Kb, ne * sizeof(double), d2h, s));
+    long long rng[4 * N_COMBOS];
     unsigned long long cnt[2 * N_COMBOS], stv[4 * N_COMBOS];
     const bool want_stats = st && !(args->flags & HFX_JK_NO_STATS);
     if (want_stats) {
-        HIPCHK(hipMemcpyAsync(cnt, e->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(rng, e->d_ranges, sizeof(rng), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(stv, e->d_stats, sizeof(stv), hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -1185,6 +1238,10 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         st->device_ms = ms;
         if (want_stats) {
+            for (int k = 0; k < N_COMBOS; ++k) {
+                cnt[2 * k] = (unsigned long long)(rng[2 * k + 1] - rng[2 * k]);
+                cnt[2 * k + 1] = (unsigned long long)(rng[2 * N_COMBOS + 2 * k + 1] - rng[2 * N_COMBOS + 2 * k]);
+            }
             double flop = 0.0;
             for (int k = 0; k < N_COMBOS; ++k) {
                 const long long c = (long long)(cnt[2 * k] + cnt[2 * k + 1]);

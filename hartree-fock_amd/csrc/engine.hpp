@@ -6,8 +6,9 @@
 //                (Cartesian normalization ratio N(p,c)/N(p,0), independent of p)
 //   shell pairs  pair_a, pair_b (la >= lb; a >= b when la == lb), pair_ps/pair_np (range of
 //                the pair's primitive pairs)
-//   prim pairs   PrimPair64 / PrimPair32, 96 / 48 bytes each, one per primitive pair of
-//                every shell pair, pair-major
+//   prim pairs   PrimPair64 / PrimPair32, 96 / 48 bytes each, pair-major; a primitive pair
+//                whose |K c_a c_b g_a g_b| is below 1e-20 (FP64 list) or 1e-11 (FP32 list)
+//                contributes nothing at that precision and is left out
 //   work lists   uint2 {bra pair, ket pair} per surviving shell quartet, one region per
 //                quartet class: FP64 entries fill the region from its start, FP32 entries
 //                from its end
@@ -48,8 +49,10 @@ constexpr int N_COMBOS = 21;
 struct EriArgs {
     const int* pair_a;
     const int* pair_b;
-    const int* pair_ps;
+    const int* pair_ps;    // primitive pairs of each shell pair kept for FP64 work
     const int* pair_np;
+    const int* pair_ps32;  // the (more strongly pruned) primitive pairs for FP32 work
+    const int* pair_np32;
     const PrimPair64* pp64;
     const PrimPair32* pp32;
     const int* sh_ao;

@@ -85,8 +85,10 @@ template <class R, int C0>
 __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab) {
     const uint2 e = a.list[qi];
     const int pid = (int)e.x, qid = (int)e.y;
-    const int ps = a.pair_ps[pid], np = valid ? a.pair_np[pid] : 0;
-    const int qs = a.pair_ps[qid], nq = valid ? a.pair_np[qid] : 0;
+    const int* __restrict__ pps = sizeof(R) == 8 ? a.pair_ps : a.pair_ps32;
+    const int* __restrict__ pnp = sizeof(R) == 8 ? a.pair_np : a.pair_np32;
+    const int ps = pps[pid], np = valid ? pnp[pid] : 0;
+    const int qs = pps[qid], nq = valid ? pnp[qid] : 0;
     const PrimPair<R>* __restrict__ pp = pp_of<R>(a);
 
     R acc[CB][NCD];
@@ -101,8 +103,11 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
         hermite_e<LC, LD>(B.oo2p, B.pax, B.pbx, Fx);
         hermite_e<LC, LD>(B.oo2p, B.pay, B.pby, Fy);
         hermite_e<LC, LD>(B.oo2p, B.paz, B.pbz, Fz);
+        // The next bra primitive pair is requested one iteration ahead to hide its latency.
+        PrimPair<R> A;
+        if (np > 0) A = pp[ps];
         for (int ip = 0; ip < np; ++ip) {
-            const PrimPair<R> A = pp[ps + ip];
+            const PrimPair<R> An = pp[ps + (ip + 1 < np ? ip + 1 : ip)];
             Ecoef<LA, LB, R> Ex, Ey, Ez;
             hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
             hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
@@ -168,6 +173,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                     });
                 }
             });
+            A = An;
         }
     }
 
@@ -327,7 +333,8 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
     }
 }
 
-template <class R, int C0>
+// The class index is a template argument only so that profiles name each class's kernels.
+template <int KC, class R, int C0>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     using S = BoysSlice<LTOT, R>;
     __shared__ R tab[BOYS_POINTS * S::nord];
@@ -337,8 +344,10 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     if (first >= n) return;  // the whole block has no work: uniform exit before the barrier
     load_boys<LTOT, R>(boys_of<R>(a), tab);
     __syncthreads();
+    // Grid-stride over the list, which is ordered heaviest-first, so every block sees the
+    // same mix of long and short quartets.  The trip count is block-uniform: every lane
+    // reaches the wavefront reductions of the digest.
     const long long stride = (long long)gridDim.x * blockDim.x;
-    // Block-uniform trip count: every lane reaches the wavefront reductions of the digest.
     for (long long base = first; base < n; base += stride) {
         const long long w = base + threadIdx.x;
         const bool valid = w < n;
@@ -359,7 +368,7 @@ template <class R>
 void launch_part(const EriArgs& a, int grid, hipStream_t s) {
     sfor_host<(CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0)>([&](auto ch) {
         constexpr int C0 = (CH_BEGIN + decltype(ch)::value) * CB;
-        hipLaunchKernelGGL((k_eri<R, C0>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_eri<K_COMBO, R, C0>), dim3(grid), dim3(256), 0, s, a);
     });
 }
 
