@@ -5,11 +5,21 @@ Cartesian functions, 246 shells), seeded +-0.02 Bohr jitter.  One step = one ful
 build (one hfx_jk call, dD = D_conv, D_prev = 0) from device-resident inputs.  The run:
   1. FP64 SCF (Schwarz 1e-10, density 1e-10, DIIS 6, energy tolerance 1e-8) -> D_conv, E64
   2. autotuned FP32 cut at D_conv, then the mixed-precision SCF with a final FP64 restart
-  3. W warm-up + K timed FP64 builds, then W + K timed mixed builds; HIP events time every
-     class kernel group inside the timed builds (roofline of the dominant one)
-  4. CPU baseline: the oracle's OpenMP port (reference release flags) on a bounded slice
+  3. W warm-up + K timed builds each of: FP64 eager, FP64 graph-replayed, mixed eager,
+     mixed graph-replayed (the headline value); then K more builds of each precision with
+     HIP events around every class kernel group (per-class times, roofline of the
+     dominant group)
+  4. batched mode: 64 seeded water clusters (n ~ U{1..8}, 6-31G) through the batched SCF
+     driver, FP64 and mixed + graph, per-molecule |dE| of mixed against FP64
+  5. gradient: RHF nuclear gradient of (H2O)8 / 6-31G**, FP64 and mixed
+  6. CPU baseline: the oracle's OpenMP port (reference release flags) on a bounded slice
      of the same build, scaled by the model FLOPs of the slice
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Profiling aid: --save-density PATH stores the converged FP64 density; a later
+  python bench.py --profile-builds --density PATH --cut C --steps K --warmup W
+runs only the W + K mixed builds with per-class HIP-event timing (no SCF), so a profiler's
+per-kernel averages cover exactly the launches the roofline is computed from.
 """
 import argparse
 import json
@@ -33,9 +43,9 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def timed_builds(L, eng, bufs, n, steps, warmup, precision, cut, Q):
+def timed_builds(L, eng, bufs, n, steps, warmup, precision, cut, Q, flags=lib.HFX_JK_CLASS_TIMING):
     dDt, dDa, J, Ka = bufs
-    kw = dict(Q=Q, precision=precision, fp32_cut=cut, flags=lib.HFX_JK_CLASS_TIMING)
+    kw = dict(Q=Q, precision=precision, fp32_cut=cut, flags=flags)
     for _ in range(warmup):
         lib.jk_device(eng, dDt, dDa, J, Ka, **kw)
     device.synchronize()
@@ -96,6 +106,74 @@ def cpu_baseline(basis, Da, Q, total_gflop, budget_s):
                        (mid, min(nS, mid + width) - 1, nS, g, total_gflop, t))
 
 
+def batched_run(L, cut, n_mol=64):
+    mols = geometry.water_batch(n_mol=n_mol, seed=1004)
+    bases = [Basis("6-31g", Z, x) for Z, x in mols]
+    kw = dict(energy_tol=1e-8, schwarz_thresh=1e-10, density_thresh=1e-10, diis_size=6, max_iter=60)
+    t = time.perf_counter()
+    r64 = lib.scf_run_batched(L, bases, options=L.default_options(**kw))
+    t64 = time.perf_counter() - t
+    t = time.perf_counter()
+    rmx = lib.scf_run_batched(L, bases, options=L.default_options(precision=lib.HFX_MIXED, fp32_cut=cut,
+                                                                  final_fp64=1, jk_flags=lib.HFX_JK_USE_GRAPH, **kw))
+    tmx = time.perf_counter() - t
+    dE = [abs(a["e_total"] - b["e_total"]) for a, b in zip(rmx, r64)]
+    dEm = [abs(a["e_total_mixed"] - b["e_total"]) for a, b in zip(rmx, r64)]
+    return dict(molecules=n_mol, functions_min_max=[min(b.n_aos for b in bases), max(b.n_aos for b in bases)],
+                total_functions=sum(b.n_aos for b in bases), scf_wall_s_fp64=t64, scf_wall_s_mixed_graph=tmx,
+                builds_fp64=r64[0]["n_jk_builds"], builds_mixed=rmx[0]["n_jk_builds"],
+                converged=sum(a["converged"] for a in rmx), max_dE_final=max(dE), max_dE_at_fp32_stop=max(dEm))
+
+
+def gradient_run(L):
+    Z, xyz = geometry.water_cluster_grid(shape=(2, 2, 2), spacing=5.6, jitter=0.4, seed=1002)
+    b = Basis("6-31g**", Z, xyz)
+    r = lib.scf_run(L, b, options=L.default_options(energy_tol=1e-9, density_tol=1e-7))
+    P = 2 * r["D_alpha"]
+    W = lib.energy_weighted_density(r["C_alpha"], r["eps_alpha"], sum(b.Z) // 2)
+    e = lib.Engine(L, b)
+    lib.rhf_gradient(e, b, P, W)  # warm-up
+    t = time.perf_counter()
+    g64, _, s64 = lib.rhf_gradient(e, b, P, W)
+    t64 = time.perf_counter() - t
+    t = time.perf_counter()
+    gmx, _, smx = lib.rhf_gradient(e, b, P, W, precision=lib.HFX_MIXED, fp32_cut=1e-5)
+    tmx = time.perf_counter() - t
+    return dict(system="(H2O)8/6-31G** (%d functions)" % b.n_aos, seconds_fp64=t64, seconds_mixed=tmx,
+                two_electron_ms_fp64=s64["device_ms"], two_electron_ms_mixed=smx["device_ms"],
+                quartets=s64["n_quartets"], fp32_quartets=smx["n_fp32"],
+                max_dgrad_mixed=float(np.abs(gmx - g64).max()), max_abs_grad=float(np.abs(g64).max()))
+
+
+def roofline_of(stmx, cm64, cm32):
+    """Roofline of the dominant class kernel group (one class, one precision) of a build."""
+    groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
+    ms, prec, k = max(groups)
+    gfl = (stmx["class_gflop64"] if prec == "f64" else stmx["class_gflop32"])[k]
+    achieved = gfl / ms if ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
+    return dict(bound="mfma", pipe="VALU %s (MI355X peak listed for %s)" % (prec, prec), kernel="k_eri %s %s" % (
+        CLASS_NAMES[k], prec), achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
+        frac=achieved / PEAK_TFLOPS[prec], traffic=None, launch_ms=ms, launch_model_gflop=gfl,
+        build_achieved_tflops=stmx["model_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
+
+
+def profile_builds(args):
+    L = lib.load()
+    Z, xyz = geometry.alkane_chain(n_carbon=args.carbons, seed=1003)
+    basis = Basis("cc-pvdz-expanded", Z, xyz)
+    n = basis.n_aos
+    Da = np.load(args.density)
+    eng = lib.Engine(L, basis)
+    Q = eng.schwarz()
+    bufs = (device.DeviceArray.from_host(2 * Da), device.DeviceArray.from_host(Da), device.DeviceArray(n * n),
+            device.DeviceArray(n * n))
+    _, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, args.cut, Q)
+    roof = roofline_of(stmx, cm64, cm32)
+    per_class = {CLASS_NAMES[i]: [round(cm64[i], 4), round(cm32[i], 4)] for i in range(21) if cm64[i] + cm32[i] > 0}
+    print(json.dumps({"profile_builds": args.steps + args.warmup, "fp32_cut": args.cut, "roofline": roof,
+                      "class_ms_f64_f32": per_class, "device_ms": stmx["device_ms"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,11 +182,21 @@ def main():
     ap.add_argument("--carbons", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-density", default=None)
+    ap.add_argument("--density", default=None)
+    ap.add_argument("--cut", type=float, default=None)
+    ap.add_argument("--profile-builds", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path (the shell-quartet work "
               "list would be sharded with one all-reduce of J/K per build); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.profile_builds:
+        if not args.density or args.cut is None:
+            print("--profile-builds needs --density and --cut", file=sys.stderr)
+            sys.exit(2)
+        profile_builds(args)
+        return
 
     L = lib.load()
     Z, xyz = geometry.alkane_chain(n_carbon=args.carbons, seed=1003)
@@ -121,6 +209,8 @@ def main():
     log("FP64 SCF: E=%.10f it=%d conv=%d wall=%.2fs (J/K %.2fs, diag %.2fs)" % (
         r64["e_total"], r64["iterations"], r64["converged"], r64["time_total_s"], r64["time_jk_s"], r64["time_diag_s"]))
     Da = r64["D_alpha"]
+    if args.save_density:
+        np.save(args.save_density, Da)
 
     eng = lib.Engine(L, basis)
     Q = eng.schwarz()
@@ -140,10 +230,18 @@ def main():
 
     bufs = (device.DeviceArray.from_host(2 * Da), device.DeviceArray.from_host(Da), device.DeviceArray(n * n),
             device.DeviceArray(n * n))
-    t64, st64, c64, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q)
+    plain = lib.HFX_JK_NO_STATS
+    graph = lib.HFX_JK_NO_STATS | lib.HFX_JK_USE_GRAPH
+    t64e, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q, plain)
+    t64, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q, graph)
     J64, K64 = bufs[2].download(), bufs[3].download()
-    tmx, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q)
+    tmxe, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q, plain)
+    tmx, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q, graph)
     Jmx, Kmx = bufs[2].download(), bufs[3].download()
+    log("builds/s: FP64 eager %.3f graph %.3f | mixed eager %.3f graph %.3f" % (
+        args.steps / t64e, args.steps / t64, args.steps / tmxe, args.steps / tmx))
+    _, st64, c64, _ = timed_builds(L, eng, bufs, n, args.steps, 1, lib.HFX_FP64, 0.0, Q)
+    _, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, 1, lib.HFX_MIXED, cut, Q)
     dF_build = float(max(np.abs(Jmx - J64).max(), np.abs(Kmx - K64).max()))
     dE_build = abs(float(np.sum(Da.reshape(-1, order="F") * ((Jmx - Kmx) - (J64 - K64)))))
     b64 = args.steps / t64
@@ -163,15 +261,13 @@ def main():
                 CLASS_NAMES[i], cm64[i], stmx["class_gflop64"][i], cm32[i], stmx["class_gflop32"][i]))
 
     # dominant kernel group of the headline (mixed) build
-    groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
-    ms, prec, k = max(groups)
-    gfl = (stmx["class_gflop64"] if prec == "f64" else stmx["class_gflop32"])[k]
-    achieved = gfl / ms if ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
-    roof = dict(bound="mfma", pipe="VALU %s (MI355X peak listed for %s)" % (prec, prec), kernel="k_eri %s %s" % (
-        CLASS_NAMES[k], prec), achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
-        frac=achieved / PEAK_TFLOPS[prec], traffic=None, launch_ms=ms, launch_model_gflop=gfl,
-        build_achieved_tflops=stmx["model_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
+    roof = roofline_of(stmx, cm64, cm32)
     per_class = {CLASS_NAMES[i]: [round(cm64[i], 3), round(cm32[i], 3)] for i in range(21) if cm64[i] + cm32[i] > 0}
+
+    batched = batched_run(L, cut)
+    log("batched: %s" % batched)
+    grad = gradient_run(L)
+    log("gradient: %s" % grad)
 
     cpu = None
     if not args.no_cpu:
@@ -196,6 +292,8 @@ def main():
                                    args.carbons, 2 * args.carbons + 2, n),
                    "fp32_cut": cut, "quartets": st64["n_quartets"], "model_gflop": st64["model_gflop"]},
         "fp64_builds_per_s": b64,
+        "fp64_eager_builds_per_s": args.steps / t64e,
+        "mixed_eager_builds_per_s": args.steps / tmxe,
         "speedup_mixed_vs_fp64": bmx / b64,
         "scf_wall_s_fp64": r64["time_total_s"],
         "scf_wall_s_mixed": rmx["time_total_s"],
@@ -209,6 +307,8 @@ def main():
         "build_max_dJK_mixed": dF_build,
         "class_ms_f64_f32": per_class,
         "class_ms_fp64_build": {CLASS_NAMES[i]: round(c64[i], 3) for i in range(21) if c64[i] > 0},
+        "batched": batched,
+        "gradient": grad,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -2,11 +2,13 @@
 
 Translation units:
   csrc/engine.hip             host engine, screening / one-electron / symmetrization kernels
+  csrc/k_grad.hip             RHF nuclear-gradient kernels
   csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
                               are fully unrolled, so they are compiled in parallel slices
   csrc/hfx_scf.cpp            host SCF driver (g++ with OpenMP)
 
-Usage: python build.py [-j N]
+Usage: python build.py [-j N] [--only engine hfx_scf ...]
+  --only recompiles just the named objects and links them with the existing others.
 """
 import argparse
 import concurrent.futures as cf
@@ -63,6 +65,7 @@ def combo_classes():
 def jobs():
     srcs = []
     srcs.append(("engine", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "engine.hip")]))
+    srcs.append(("k_grad", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "k_grad.hip")]))
     for (k, la, lb, lc, ld, nch) in combo_classes():
         for b in range(0, nch, CHUNKS_PER_TU):
             e = min(nch, b + CHUNKS_PER_TU)
@@ -82,10 +85,17 @@ def _run(name, cmd):
     return name, obj, r.returncode, msg, time.time() - t
 
 
-def build(jobs_n=None, verbose=False):
+def build(jobs_n=None, verbose=False, only=None):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
     todo = jobs()
+    keep = []
+    if only:
+        keep = [os.path.join(BUILD, name + ".o") for name, _ in todo if not name.startswith(tuple(only))]
+        missing = [o for o in keep if not os.path.exists(o)]
+        if missing:
+            raise RuntimeError("--only needs a full build first; missing " + ", ".join(missing[:4]))
+        todo = [j for j in todo if j[0].startswith(tuple(only))]
     n = jobs_n or min(16, os.cpu_count() or 8)
     objs, failed = [], []
     t0 = time.time()
@@ -105,7 +115,7 @@ def build(jobs_n=None, verbose=False):
         for name, msg in failed:
             print("---- %s ----\n%s" % (name, msg[-4000:]), file=sys.stderr)
         raise RuntimeError("HIP build failed: " + ", ".join(n for n, _ in failed))
-    objs.sort()
+    objs = sorted(objs + keep)
     link = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + \
         ["-lgomp", "-Wl,-Bsymbolic", "-Wl,--no-undefined"]
     r = subprocess.run(link, capture_output=True, text=True)
@@ -120,5 +130,6 @@ def build(jobs_n=None, verbose=False):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--only", nargs="*", default=None)
     a = ap.parse_args()
-    build(a.j, verbose=True)
+    build(a.j, verbose=True, only=a.only)

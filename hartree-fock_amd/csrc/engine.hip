@@ -129,6 +129,11 @@ __global__ void k_fill(double* p, long long n, double v) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = v;
 }
+__global__ void k_scale(const double* __restrict__ in, double* __restrict__ out, long long n, double f) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = f * in[i];
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
@@ -534,6 +539,16 @@ void launch_one(int pc, const OneArgs& o, hipStream_t s) {
     }
 }
 
+// Build parameters that kernels of the pipeline take by value (the graph cache key).
+struct BuildParams {
+    double tauS, tauD, cut;
+    bool mixed, uhf, timing;
+    bool operator==(const BuildParams& o) const {
+        return tauS == o.tauS && tauD == o.tauD && cut == o.cut && mixed == o.mixed && uhf == o.uhf &&
+               timing == o.timing;
+    }
+};
+
 // ---------------------------------------------------------------------------------------
 // Device memory owned by an engine.
 // ---------------------------------------------------------------------------------------
@@ -599,6 +614,14 @@ struct hfx_engine {
     double* d_pair_q = nullptr;
     PrimPair64* d_pp64 = nullptr;
     PrimPair32* d_pp32 = nullptr;
+    double* d_pab64 = nullptr;  // exponents (a, b) per primitive pair (gradient kernels)
+    float* d_pab32 = nullptr;
+    int* d_sh_l = nullptr;
+    // gradient buffers, grown on demand
+    int* d_sh_atom = nullptr;
+    double *d_atom_z = nullptr, *d_atom_xyz = nullptr, *d_gpart = nullptr, *d_gout = nullptr;
+    int atom_cap = 0;
+    long long gpart_cap = 0;
     int *d_sh_ao = nullptr, *d_sh_nao = nullptr, *d_sh_mn = nullptr, *d_sh_local = nullptr, *d_sh_ns = nullptr;
     int *d_sh_po = nullptr, *d_sh_np = nullptr, *d_sh_no = nullptr, *d_sh_mol = nullptr;
     long long *d_sh_mbase = nullptr, *d_sh_qbase = nullptr;
@@ -629,9 +652,14 @@ struct hfx_engine {
     size_t scan_tmp_bytes = 0;
     unsigned long long* d_stats = nullptr;  // [4 * N_COMBOS]
     hipEvent_t cev[2 * N_COMBOS + 1] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    BuildParams graph_key{-1.0, -1.0, -1.0, false, false, false};
     long long* d_ranges = nullptr;          // [4*N_COMBOS]
 
     ~hfx_engine() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (hipEvent_t& x : cev)
             if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -734,6 +762,8 @@ int sort_pairs(hfx_engine* e, const std::vector<double>& qt) {
     }
     HIPCHK(hipMemcpy(e->d_pair_q, e->pair_q.data(), e->n_pairs * sizeof(double), hipMemcpyHostToDevice));
     e->sort_q = qt;
+    // A captured build baked in the previous order's launch shapes: capture again.
+    e->graph_key.tauS = -1.0;
     return HFX_OK;
 }
 
@@ -907,6 +937,8 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     constexpr double PRUNE64 = 1e-20, PRUNE32 = 1e-11;
     std::vector<PrimPair64> pp64;
     std::vector<PrimPair32> pp32;
+    std::vector<double> pab64;
+    std::vector<float> pab32;
     std::vector<int> cls_count(N_PAIR_CLASSES, 0);
     for (int m = 0; m < n_mol; ++m) {
         const int s0 = e->mol_shell0[m], s1 = s0 + e->mol_nS[m];
@@ -950,7 +982,15 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
                         const double gb = e->norm[e->sh_no[b] + pb * e->sh_nao[b]];
                         const double kc = K * (e->coeff[e->sh_po[a] + pa] * ga) * (e->coeff[e->sh_po[b] + pb] * gb);
                         d.kp = kc * oop;
-                        if (std::fabs(kc) >= PRUNE64) pp64.push_back(d);
+                        if (std::fabs(kc) >= PRUNE64) {
+                            pp64.push_back(d);
+                            pab64.push_back(al);
+                            pab64.push_back(be);
+                        }
+                        if (std::fabs(kc) >= PRUNE32) {
+                            pab32.push_back((float)al);
+                            pab32.push_back((float)be);
+                        }
                         if (std::fabs(kc) >= PRUNE32)
                             pp32.push_back(PrimPair32{(float)d.p, (float)d.px, (float)d.py, (float)d.pz, (float)d.kp,
                                                       (float)d.pax, (float)d.pay, (float)d.paz, (float)d.pbx,
@@ -1014,7 +1054,9 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
         (rc = upload(e.get(), e->d_alpha, e->alpha)) || (rc = upload(e.get(), e->d_coeff, e->coeff)) ||
         (rc = upload(e.get(), e->d_norm, e->norm)) || (rc = upload(e.get(), e->d_boys64, b64)) ||
         (rc = upload(e.get(), e->d_boys32, b32)) || (rc = upload(e.get(), e->d_mbase, e->mol_mbase)) ||
-        (rc = upload(e.get(), e->d_mn, e->mol_N)) || (rc = upload(e.get(), e->d_reg, e->reg)))
+        (rc = upload(e.get(), e->d_mn, e->mol_N)) || (rc = upload(e.get(), e->d_reg, e->reg)) ||
+        (rc = upload(e.get(), e->d_pab64, pab64)) || (rc = upload(e.get(), e->d_pab32, pab32)) ||
+        (rc = upload(e.get(), e->d_sh_l, e->sh_l)))
         return rc;
     const long long ne = e->n_elems, nq = e->n_qelems;
     e->d_Dt = e->mem.alloc<double>(ne);
@@ -1062,6 +1104,150 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     return HFX_OK;
 }
 
+// The device pipeline of one build, from the staged densities (d_Dt, d_Da, d_Db) to the
+// symmetrized outputs (d_J, d_Ka, d_Kb).  No host synchronization, no memset: it is
+// captured into a hipGraph as is.  With P.timing, events are recorded after every class
+// kernel group (ev_slot[2k + precision] = event index, -1 when absent).
+// Screening of a build: shell-pair density maxima of the staged densities, the two
+// counting/writing passes over every class and the per-class list ranges (d_ranges).
+int enqueue_screen(hfx_engine* e, const BuildParams& P, hipStream_t s) {
+    const bool uhf = P.uhf, mixed = P.mixed;
+
+    // shell-pair density maxima
+    {
+        DmaxArgs d{};
+        d.pair_a = e->d_pair_a;
+        d.pair_b = e->d_pair_b;
+        d.sh_ao = e->d_sh_ao;
+        d.sh_nao = e->d_sh_nao;
+        d.sh_mbase = e->d_sh_mbase;
+        d.sh_mn = e->d_sh_mn;
+        d.sh_local = e->d_sh_local;
+        d.sh_qbase = e->d_sh_qbase;
+        d.sh_ns = e->d_sh_ns;
+        d.Dt = e->d_Dt;
+        d.Da = e->d_Da;
+        d.Db = uhf ? e->d_Db : nullptr;
+        d.Mt = e->d_Mt;
+        d.Ma = e->d_Ma;
+        d.Mb = e->d_Mb;
+        d.n_pairs = e->n_pairs;
+        hipLaunchKernelGGL(k_dmax, dim3((e->n_pairs + 255) / 256), dim3(256), 0, s, d);
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::max<long long>(1, std::min<long long>(1024, (e->n_slots + 256) / 256))),
+                       dim3(256), 0, s, e->d_counts, (int)(e->n_slots + 1));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 4 * N_COMBOS);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1)
+            HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
+                                                    (int)(e->n_slots + 1), s));
+    for (int k = 0; k < N_COMBOS; ++k) {
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
+        ScreenArgs a{};
+        a.bra_ids = e->d_cls_ids[p1];
+        a.n_bra = (int)e->cls_ids[p1].size();
+        a.ket_ids = e->d_cls_ids[p2];
+        a.ket_q = e->d_cls_q[p2];
+        a.ket_seg_start = e->d_seg_start[p2];
+        a.ket_seg_end = e->d_seg_end[p2];
+        a.ket_molseg = e->d_molseg[p2];
+        a.diagonal = (p1 == p2);
+        a.pair_info = e->d_pair_info;
+        a.pair_mol = e->d_pair_mol;
+        a.pair_q = e->d_pair_q;
+        a.sh_qbase = e->d_sh_qbase;
+        a.sh_ns = e->d_sh_ns;
+        a.Mt = e->d_Mt;
+        a.Ma = e->d_Ma;
+        a.Mb = uhf ? e->d_Mb : nullptr;
+        a.ns_max = e->ns_max;
+        a.tauS = P.tauS;
+        a.tauD = P.tauD;
+        a.cut = P.cut;
+        a.mixed = mixed;
+        a.nS_total = e->nS;
+        a.list = e->d_list;
+        a.counts = e->d_counts;
+        a.offs = e->d_offs;
+        a.slot64 = e->slots[3 * k];
+        a.slot32 = e->slots[3 * k + 1];
+        a.stats = e->d_stats + 4 * k;
+        const size_t row_bytes = (size_t)(uhf ? 4 : 2) * e->ns_max * sizeof(double);
+        const int wpb = (int)std::min<size_t>(4, (60 * 1024) / row_bytes);
+        if (wpb < 1) return fail(HFX_ERR_UNSUPPORTED, "molecule too large for the screening kernel's LDS rows");
+        const dim3 sg((a.n_bra + wpb - 1) / wpb), sb(64 * wpb);
+        if (pass == 0)
+            hipLaunchKernelGGL(k_screen<0>, sg, sb, wpb * row_bytes, s, a);
+        else
+            hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
+    }
+    }
+    hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_offs, e->d_slots, e->d_ranges);
+    return HFX_OK;
+}
+
+int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_slot, int& n_ev) {
+    const bool uhf = P.uhf, mixed = P.mixed, timing = P.timing;
+    const long long ne = e->n_elems;
+    n_ev = 0;
+    {
+        const int rc = enqueue_screen(e, P, s);
+        if (rc) return rc;
+    }
+    {
+        const int g = e->n_cu * 4;
+        const long long nc = ne * hfx_engine::ACC_COPIES;
+        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, nc, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, nc, 0.0);
+        if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, nc, 0.0);
+    }
+    EriArgs ea = eri_args(e);
+    ea.list = e->d_list;
+    ea.Dt = e->d_Dt;
+    ea.Da = e->d_Da;
+    ea.Db = uhf ? e->d_Db : nullptr;
+    ea.AJ = e->d_AJ;
+    ea.AKa = e->d_AKa;
+    ea.AKb = e->d_AKb;
+    ea.acc_stride = ne;
+    ea.n_copies = hfx_engine::ACC_COPIES;
+    ea.tauS = P.tauS;
+    ea.out = nullptr;
+    ea.out_off = nullptr;
+    const int grid = eri_grid(e);
+    if (timing) HIPCHK(hipEventRecord(e->cev[n_ev++], s));
+    for (int k = 0; k < N_COMBOS; ++k) {
+        ev_slot[2 * k] = ev_slot[2 * k + 1] = -1;
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
+        ea.range = e->d_ranges + 2 * k;
+        launch_eri(k, 0, ea, grid, s);
+        if (timing) {
+            HIPCHK(hipEventRecord(e->cev[n_ev], s));
+            ev_slot[2 * k] = n_ev++;
+        }
+        if (mixed) {
+            ea.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
+            launch_eri(k, 1, ea, grid, s);
+            if (timing) {
+                HIPCHK(hipEventRecord(e->cev[n_ev], s));
+                ev_slot[2 * k + 1] = n_ev++;
+            }
+        }
+    }
+    {
+        const dim3 g(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256))), e->n_mol);
+        const int cp = hfx_engine::ACC_COPIES;
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, ne, cp, e->d_J, e->d_mbase, e->d_mn);
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, ne, cp, e->d_Ka, e->d_mbase, e->d_mn);
+        if (uhf) hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, ne, cp, e->d_Kb, e->d_mbase, e->d_mn);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipGetLastError());
+    return HFX_OK;
+}
+
 int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     if (!e || !args || !args->dD_tot || !args->dD_alpha || !J || !Ka)
         return fail(HFX_ERR_INVALID, "hfx_jk: null argument");
@@ -1095,131 +1281,36 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     HIPCHK(hipMemcpyAsync(e->d_Da, args->dD_alpha, ne * sizeof(double), h2d, s));
     if (uhf) HIPCHK(hipMemcpyAsync(e->d_Db, args->dD_beta, ne * sizeof(double), h2d, s));
     HIPCHK(hipEventRecord(e->ev0, s));
-
-    // shell-pair density maxima
-    {
-        DmaxArgs d{};
-        d.pair_a = e->d_pair_a;
-        d.pair_b = e->d_pair_b;
-        d.sh_ao = e->d_sh_ao;
-        d.sh_nao = e->d_sh_nao;
-        d.sh_mbase = e->d_sh_mbase;
-        d.sh_mn = e->d_sh_mn;
-        d.sh_local = e->d_sh_local;
-        d.sh_qbase = e->d_sh_qbase;
-        d.sh_ns = e->d_sh_ns;
-        d.Dt = e->d_Dt;
-        d.Da = e->d_Da;
-        d.Db = uhf ? e->d_Db : nullptr;
-        d.Mt = e->d_Mt;
-        d.Ma = e->d_Ma;
-        d.Mb = e->d_Mb;
-        d.n_pairs = e->n_pairs;
-        hipLaunchKernelGGL(k_dmax, dim3((e->n_pairs + 255) / 256), dim3(256), 0, s, d);
-    }
-    hipLaunchKernelGGL(k_fill_u64, dim3(std::max<long long>(1, std::min<long long>(1024, (e->n_slots + 256) / 256))),
-                       dim3(256), 0, s, e->d_counts, (int)(e->n_slots + 1));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 4 * N_COMBOS);
-    const bool mixed = args->precision == HFX_MIXED && args->fp32_cut > 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1)
-            HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
-                                                    (int)(e->n_slots + 1), s));
-    for (int k = 0; k < N_COMBOS; ++k) {
-        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
-        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
-        ScreenArgs a{};
-        a.bra_ids = e->d_cls_ids[p1];
-        a.n_bra = (int)e->cls_ids[p1].size();
-        a.ket_ids = e->d_cls_ids[p2];
-        a.ket_q = e->d_cls_q[p2];
-        a.ket_seg_start = e->d_seg_start[p2];
-        a.ket_seg_end = e->d_seg_end[p2];
-        a.ket_molseg = e->d_molseg[p2];
-        a.diagonal = (p1 == p2);
-        a.pair_info = e->d_pair_info;
-        a.pair_mol = e->d_pair_mol;
-        a.pair_q = e->d_pair_q;
-        a.sh_qbase = e->d_sh_qbase;
-        a.sh_ns = e->d_sh_ns;
-        a.Mt = e->d_Mt;
-        a.Ma = e->d_Ma;
-        a.Mb = uhf ? e->d_Mb : nullptr;
-        a.ns_max = e->ns_max;
-        a.tauS = args->schwarz_thresh;
-        a.tauD = args->density_thresh;
-        a.cut = args->fp32_cut;
-        a.mixed = mixed;
-        a.nS_total = e->nS;
-        a.list = e->d_list;
-        a.counts = e->d_counts;
-        a.offs = e->d_offs;
-        a.slot64 = e->slots[3 * k];
-        a.slot32 = e->slots[3 * k + 1];
-        a.stats = e->d_stats + 4 * k;
-        const size_t row_bytes = (size_t)(uhf ? 4 : 2) * e->ns_max * sizeof(double);
-        const int wpb = (int)std::min<size_t>(4, (60 * 1024) / row_bytes);
-        if (wpb < 1) return fail(HFX_ERR_UNSUPPORTED, "molecule too large for the screening kernel's LDS rows");
-        const dim3 sg((a.n_bra + wpb - 1) / wpb), sb(64 * wpb);
-        if (pass == 0)
-            hipLaunchKernelGGL(k_screen<0>, sg, sb, wpb * row_bytes, s, a);
-        else
-            hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
-    }
-    }
-    hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_offs, e->d_slots, e->d_ranges);
-    {
-        const int g = e->n_cu * 4;
-        const long long nc = ne * hfx_engine::ACC_COPIES;
-        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, nc, 0.0);
-        hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, nc, 0.0);
-        if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, nc, 0.0);
-    }
-    EriArgs ea = eri_args(e);
-    ea.list = e->d_list;
-    ea.Dt = e->d_Dt;
-    ea.Da = e->d_Da;
-    ea.Db = uhf ? e->d_Db : nullptr;
-    ea.AJ = e->d_AJ;
-    ea.AKa = e->d_AKa;
-    ea.AKb = e->d_AKb;
-    ea.acc_stride = ne;
-    ea.n_copies = hfx_engine::ACC_COPIES;
-    ea.tauS = args->schwarz_thresh;
-    ea.out = nullptr;
-    ea.out_off = nullptr;
-    const int grid = eri_grid(e);
-    const bool timing = (args->flags & HFX_JK_CLASS_TIMING) != 0;
-    int ev_slot[2 * N_COMBOS];  // event index after each (class, precision) group, -1 if absent
+    BuildParams P{args->schwarz_thresh, args->density_thresh, args->fp32_cut,
+                  args->precision == HFX_MIXED && args->fp32_cut > 0.0, uhf,
+                  (args->flags & HFX_JK_CLASS_TIMING) != 0};
+    int ev_slot[2 * N_COMBOS];
     int n_ev = 0;
-    if (timing) HIPCHK(hipEventRecord(e->cev[n_ev++], s));
-    for (int k = 0; k < N_COMBOS; ++k) {
-        ev_slot[2 * k] = ev_slot[2 * k + 1] = -1;
-        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
-        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
-        ea.range = e->d_ranges + 2 * k;
-        launch_eri(k, 0, ea, grid, s);
-        if (timing) {
-            HIPCHK(hipEventRecord(e->cev[n_ev], s));
-            ev_slot[2 * k] = n_ev++;
-        }
-        if (mixed) {
-            ea.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
-            launch_eri(k, 1, ea, grid, s);
-            if (timing) {
-                HIPCHK(hipEventRecord(e->cev[n_ev], s));
-                ev_slot[2 * k + 1] = n_ev++;
+    const bool timing = P.timing;
+    if ((args->flags & HFX_JK_USE_GRAPH) && !timing) {
+        if (!e->graph_exec || !(e->graph_key == P)) {
+            if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+            if (e->graph) (void)hipGraphDestroy(e->graph);
+            e->graph_exec = nullptr;
+            e->graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_build(e, P, s, ev_slot, n_ev);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
             }
+            if (ce != hipSuccess) return fail(HFX_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+            e->graph = g;
+            HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+            e->graph_key = P;
         }
+        HIPCHK(hipGraphLaunch(e->graph_exec, s));
+    } else {
+        const int rc = enqueue_build(e, P, s, ev_slot, n_ev);
+        if (rc) return rc;
     }
-    {
-        const dim3 g(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256))), e->n_mol);
-        const int cp = hfx_engine::ACC_COPIES;
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, ne, cp, e->d_J, e->d_mbase, e->d_mn);
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, ne, cp, e->d_Ka, e->d_mbase, e->d_mn);
-        if (uhf) hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, ne, cp, e->d_Kb, e->d_mbase, e->d_mn);
-    }
-    HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e->ev1, s));
     HIPCHK(hipMemcpyAsync(J, e->d_J, ne * sizeof(double), d2h, s));
     HIPCHK(hipMemcpyAsync(Ka, e->d_Ka, ne * sizeof(double), d2h, s));
@@ -1270,6 +1361,165 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
                 prev = ev_slot[x];
             }
         }
+    }
+    return HFX_OK;
+}
+
+// The RHF gradient: screening of the J/K build on (P, P/2), the two-electron kernels on
+// the surviving quartets, the one-electron kernel on every shell pair, the reduction of
+// the per-block rows; the nuclear repulsion term is added on the host.
+int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts, hfx_grad_stats* st) {
+    if (!e || !a || !grad || !a->P || !a->W || a->n_atoms <= 0 || !a->Z || !a->xyz)
+        return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: null argument");
+    if (e->n_mol != 1) return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: single-molecule engines only");
+    if (a->precision != HFX_FP64 && a->precision != HFX_MIXED)
+        return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: unknown precision");
+    const bool devp = (a->flags & HFX_JK_DEVICE_PTRS) != 0;
+    const long long ne = e->n_elems;
+    const int nat = a->n_atoms, n3 = 3 * nat;
+    if (!devp && (!finite_all(a->P, ne) || !finite_all(a->W, ne)))
+        return fail(HFX_ERR_NONFINITE, "hfx_rhf_gradient: non-finite density");
+    if (!finite_all(a->Z, nat) || !finite_all(a->xyz, n3)) return fail(HFX_ERR_NONFINITE, "hfx_rhf_gradient: atoms");
+    std::vector<int> sat(e->nS);
+    for (int sh = 0; sh < e->nS; ++sh) {
+        if (a->shell_atom) {
+            sat[sh] = a->shell_atom[sh];
+        } else {
+            sat[sh] = -1;
+            for (int A = 0; A < nat && sat[sh] < 0; ++A)
+                if (e->sh_ctr[3 * sh] == a->xyz[3 * A] && e->sh_ctr[3 * sh + 1] == a->xyz[3 * A + 1] &&
+                    e->sh_ctr[3 * sh + 2] == a->xyz[3 * A + 2])
+                    sat[sh] = A;
+        }
+        if (sat[sh] < 0 || sat[sh] >= nat) return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: shell not on an atom");
+    }
+    // Engine-owned gradient buffers, grown on demand.
+    const int grid = e->n_cu * 8;
+    const long long stride = ((3LL * n3 + 31) / 32) * 32;
+    if (!e->d_sh_atom) {
+        e->d_sh_atom = e->mem.alloc<int>(e->nS);
+        if (!e->d_sh_atom) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+    }
+    if (nat > e->atom_cap) {
+        e->d_atom_z = e->mem.alloc<double>(nat);
+        e->d_atom_xyz = e->mem.alloc<double>(3LL * nat);
+        e->d_gout = e->mem.alloc<double>(3LL * n3);
+        if (!e->d_atom_z || !e->d_atom_xyz || !e->d_gout) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+        e->atom_cap = nat;
+    }
+    if (stride * grid > e->gpart_cap) {
+        e->d_gpart = e->mem.alloc<double>(stride * grid);
+        if (!e->d_gpart) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+        e->gpart_cap = stride * grid;
+    }
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    const hipMemcpyKind h2d = devp ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPCHK(hipMemcpyAsync(e->d_sh_atom, sat.data(), e->nS * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->d_atom_z, a->Z, nat * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->d_atom_xyz, a->xyz, n3 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(e->d_Dt, a->P, ne * sizeof(double), h2d, s));
+    HIPCHK(hipMemcpyAsync(e->d_Db, a->W, ne * sizeof(double), h2d, s));
+    hipLaunchKernelGGL(k_scale, dim3(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256)))), dim3(256), 0, s,
+                       e->d_Dt, e->d_Da, ne, 0.5);
+    hipLaunchKernelGGL(k_fill, dim3(std::max(1, std::min(e->n_cu * 4, (int)((stride * grid + 255) / 256)))), dim3(256),
+                       0, s, e->d_gpart, stride * grid, 0.0);
+    // Q order of the engine's own table
+    if (e->sort_q != e->own_q) {
+        const int rc = sort_pairs(e, e->own_q);
+        if (rc) return rc;
+    }
+    const bool mixed = a->precision == HFX_MIXED && a->fp32_cut > 0.0;
+    const BuildParams bp{a->schwarz_thresh, a->density_thresh, a->fp32_cut, mixed, false, false};
+    {
+        const int rc = enqueue_screen(e, bp, s);
+        if (rc) return rc;
+    }
+    GradArgs g{};
+    g.pair_a = e->d_pair_a;
+    g.pair_b = e->d_pair_b;
+    g.pair_ps = e->d_pair_ps;
+    g.pair_np = e->d_pair_np;
+    g.pp64 = e->d_pp64;
+    g.pab64 = e->d_pab64;
+    g.pair_ps32 = e->d_pair_ps32;
+    g.pair_np32 = e->d_pair_np32;
+    g.pp32 = e->d_pp32;
+    g.pab32 = e->d_pab32;
+    g.boys64 = e->d_boys64;
+    g.boys32 = e->d_boys32;
+    g.sh_l = e->d_sh_l;
+    g.sh_ao = e->d_sh_ao;
+    g.sh_np = e->d_sh_np;
+    g.sh_po = e->d_sh_po;
+    g.sh_no = e->d_sh_no;
+    g.sh_cf = e->d_sh_cf;
+    g.sh_atom = e->d_sh_atom;
+    g.ctr = e->d_ctr;
+    g.alpha = e->d_alpha;
+    g.coeff = e->d_coeff;
+    g.norm = e->d_norm;
+    g.atom_z = e->d_atom_z;
+    g.atom_xyz = e->d_atom_xyz;
+    g.n_atoms = nat;
+    g.n_pairs = e->n_pairs;
+    g.n_ao = e->N_tot;
+    g.P = e->d_Dt;
+    g.W = e->d_Db;
+    g.list = e->d_list;
+    g.part = e->d_gpart;
+    g.part_stride = stride;
+    g.n3 = n3;
+    HIPCHK(hipEventRecord(e->ev0, s));
+    for (int k = 0; k < N_COMBOS; ++k) {
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
+        const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
+        g.range = e->d_ranges + 2 * k;
+        launch_grad_eri(0, g, la, lb, lc, ld, grid, s);
+        if (mixed) {
+            g.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
+            launch_grad_eri(1, g, la, lb, lc, ld, grid, s);
+        }
+    }
+    HIPCHK(hipEventRecord(e->ev1, s));
+    launch_grad_one(g, grid, s);
+    launch_grad_reduce(g, grid, e->d_gout, s);
+    HIPCHK(hipGetLastError());
+    std::vector<double> out(3 * (size_t)n3);
+    long long rng[4 * N_COMBOS];
+    HIPCHK(hipMemcpyAsync(out.data(), e->d_gout, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rng, e->d_ranges, sizeof(rng), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // nuclear repulsion: d/dX_A sum_{A<B} Z_A Z_B / R_AB
+    std::vector<double> gn(n3, 0.0);
+    for (int A = 0; A < nat; ++A)
+        for (int B = 0; B < nat; ++B) {
+            if (A == B) continue;
+            const double dx = a->xyz[3 * A] - a->xyz[3 * B], dy = a->xyz[3 * A + 1] - a->xyz[3 * B + 1],
+                         dz = a->xyz[3 * A + 2] - a->xyz[3 * B + 2];
+            const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const double f = a->Z[A] * a->Z[B] / (r * r * r);
+            gn[3 * A] -= f * dx;
+            gn[3 * A + 1] -= f * dy;
+            gn[3 * A + 2] -= f * dz;
+        }
+    for (int x = 0; x < n3; ++x) grad[x] = gn[x] + out[x] + out[n3 + x] + out[2 * n3 + x];
+    if (parts) {
+        std::copy(gn.begin(), gn.end(), parts);
+        std::copy(out.begin(), out.end(), parts + n3);
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        for (int k = 0; k < N_COMBOS; ++k) {
+            st->n_quartets += rng[2 * k + 1] - rng[2 * k];
+            const long long n32 = rng[2 * N_COMBOS + 2 * k + 1] - rng[2 * N_COMBOS + 2 * k];
+            st->n_quartets += n32;
+            st->n_fp32 += n32;
+        }
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        st->device_ms = ms;
     }
     return HFX_OK;
 }
@@ -1459,6 +1709,10 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st);
+}
+
+int hfx_rhf_gradient(hfx_engine* e, const hfx_grad_args* args, double* grad, double* parts, hfx_grad_stats* stats) {
+    return grad_impl(e, args, grad, parts, stats);
 }
 
 }  // extern "C"

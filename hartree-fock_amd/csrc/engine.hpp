@@ -76,6 +76,48 @@ struct EriArgs {
     const float* boys32;
 };
 
+// Arguments of the RHF-gradient kernels (k_grad.hip).
+struct GradArgs {
+    const int* pair_a;
+    const int* pair_b;
+    const int* pair_ps;
+    const int* pair_np;
+    const PrimPair64* pp64;
+    const double* pab64;  // exponents (a, b) of every FP64 primitive pair
+    const int* pair_ps32;
+    const int* pair_np32;
+    const PrimPair32* pp32;
+    const float* pab32;
+    const double* boys64;
+    const float* boys32;
+    const int* sh_l;
+    const int* sh_ao;
+    const int* sh_np;
+    const int* sh_po;
+    const int* sh_no;
+    const double* sh_cf;
+    const int* sh_atom;
+    const double* ctr;
+    const double* alpha;
+    const double* coeff;
+    const double* norm;
+    const double* atom_z;
+    const double* atom_xyz;
+    int n_atoms;
+    int n_pairs;
+    int n_ao;
+    const double* P;  // N x N total density
+    const double* W;  // N x N energy-weighted density
+    const uint2* list;
+    const long long* range;
+    double* part;            // per-block rows: [-W dS | P dh | two-electron], n3 each
+    long long part_stride;
+    int n3;                  // 3 * n_atoms
+};
+void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, hipStream_t s);
+void launch_grad_one(const GradArgs& g, int grid, hipStream_t s);
+void launch_grad_reduce(const GradArgs& g, int n_rows, double* out, hipStream_t s);
+
 // Each kernel translation unit compiles chunks [begin, end) of one class and registers a
 // launcher for them at load time; launch_eri runs every registered part of a class.
 using EriPartFn = void (*)(int fp32, const EriArgs& a, int grid, hipStream_t s);

@@ -159,7 +159,9 @@ struct Scf {
         return 0.5 * trace_prod(n, Da.data(), ha.data()) + 0.5 * trace_prod(n, Db.data(), hb.data());
     }
 
-    int build_fock(bool full) {  // SCF::buildFockMatrixDirect (SCF.cpp:523-744) via hfx_jk
+    // The density increments of this iteration (SCF.cpp:536-541); with full, the increments
+    // restart from zero (F_prev = h, D_prev = 0), i.e. a full rebuild.
+    void deltas(bool full, double* dDt, double* dDa, double* dDb) {
         if (full) {
             std::fill(Dt_prev.begin(), Dt_prev.end(), 0.0);
             std::fill(Da_prev.begin(), Da_prev.end(), 0.0);
@@ -167,15 +169,30 @@ struct Scf {
             Fa_prev = h;
             Fb_prev = h;
         }
-        Mat dDt = mat(), dDa = mat(), dDb;
-        for (size_t x = 0; x < dDt.size(); ++x) {
+        for (size_t x = 0; x < Dt.size(); ++x) {
             dDt[x] = Dt[x] - Dt_prev[x];
             dDa[x] = Da[x] - Da_prev[x];
+            if (uhf) dDb[x] = Db[x] - Db_prev[x];
         }
+    }
+
+    // F = F_prev + J - K; F_prev = F (SCF.cpp:731-743).
+    void apply_jk(const double* J, const double* Ka, const double* Kb) {
+        for (size_t x = 0; x < Fa.size(); ++x) Fa[x] = Fa_prev[x] + J[x] - Ka[x];
+        Fa_prev = Fa;
         if (uhf) {
-            dDb = mat();
-            for (size_t x = 0; x < dDb.size(); ++x) dDb[x] = Db[x] - Db_prev[x];
+            for (size_t x = 0; x < Fb.size(); ++x) Fb[x] = Fb_prev[x] + J[x] - Kb[x];
+            Fb_prev = Fb;
+        } else {
+            Fb = Fa;
+            Fb_prev = Fa_prev;
         }
+    }
+
+    int build_fock(bool full) {  // SCF::buildFockMatrixDirect (SCF.cpp:523-744) via hfx_jk
+        Mat dDt = mat(), dDa = mat(), dDb;
+        if (uhf) dDb = mat();
+        deltas(full, dDt.data(), dDa.data(), uhf ? dDb.data() : nullptr);
         Mat J = mat(), Ka = mat(), Kb;
         if (uhf) Kb = mat();
         hfx_jk_args a{};
@@ -195,16 +212,44 @@ struct Scf {
         if (rc) return rc;
         n_builds++;
         n_quart_last = st.n_quartets;
-        for (size_t x = 0; x < J.size(); ++x) Fa[x] = Fa_prev[x] + J[x] - Ka[x];
-        Fa_prev = Fa;
-        if (uhf) {
-            for (size_t x = 0; x < J.size(); ++x) Fb[x] = Fb_prev[x] + J[x] - Kb[x];
-            Fb_prev = Fb;
-        } else {
-            Fb = Fa;
-            Fb_prev = Fa_prev;
-        }
+        apply_jk(J.data(), Ka.data(), uhf ? Kb.data() : nullptr);
         return HFX_OK;
+    }
+
+    // Everything after the Fock build in one SCF cycle (SCF.cpp:60-91): DIIS update and
+    // extrapolation on FDS - SDF, diagonalization, energy, convergence test.
+    bool after_build(Diis* diis, double last, hfx_scf_result* res, int slot) {
+        if (diis) {
+            Mat t1 = mat(), t2 = mat(), ea = mat(), eb = mat();
+            hfx_la::mm(n, Fa.data(), Da.data(), t1.data());
+            hfx_la::mm(n, t1.data(), S.data(), t2.data());
+            hfx_la::mm(n, S.data(), Da.data(), t1.data());
+            hfx_la::mm(n, t1.data(), Fa.data(), ea.data());
+            for (size_t x = 0; x < ea.size(); ++x) ea[x] = t2[x] - ea[x];
+            hfx_la::mm(n, Fb.data(), Db.data(), t1.data());
+            hfx_la::mm(n, t1.data(), S.data(), t2.data());
+            hfx_la::mm(n, S.data(), Db.data(), t1.data());
+            hfx_la::mm(n, t1.data(), Fb.data(), eb.data());
+            for (size_t x = 0; x < eb.size(); ++x) eb[x] = t2[x] - eb[x] + ea[x];
+            diis->update(Fa, Fb, eb);
+            diis->extrapolate(Fa, Fb);
+        }
+        diagonalize_and_update();
+        dE = std::abs(e_elec - last);
+        double sum = 0.0;
+        for (size_t x = 0; x < Dt.size(); ++x) {
+            const double d = Dt[x] - Dt_prev[x];
+            sum += d * d;
+        }
+        dD = std::sqrt(sum);
+        diis_err = diis ? diis->err_norm() : 0.0;
+        if (res && res->trace && slot < res->trace_cap) {
+            res->trace[4 * slot + 0] = e_elec;
+            res->trace[4 * slot + 1] = dE;
+            res->trace[4 * slot + 2] = dD;
+            res->trace[4 * slot + 3] = diis_err;
+        }
+        return dE < o.energy_tol && dD < o.density_tol && (!diis || diis_err < o.diis_error_tol);
     }
 
     void diagonalize_and_update() {  // SCF.cpp:747-835
@@ -280,53 +325,75 @@ struct Scf {
         e_elec = energy();
     }
 
-    // The SCF cycles of SCF::run (SCF.cpp:51-114); returns true when converged.
+    // The SCF cycles of SCF::run (SCF.cpp:51-114).
     int cycles(Diis* diis, bool first_full, int& it_done, bool& conv, hfx_scf_result* res, int trace_base) {
         conv = false;
         for (iteration = 0; iteration < o.max_iter; ++iteration) {
             const double last = e_elec;
             const int rc = build_fock(o.full_builds || (first_full && iteration == 0));
             if (rc) return rc;
-            if (diis) {
-                Mat t1 = mat(), t2 = mat(), ea = mat(), eb = mat();
-                hfx_la::mm(n, Fa.data(), Da.data(), t1.data());
-                hfx_la::mm(n, t1.data(), S.data(), t2.data());
-                hfx_la::mm(n, S.data(), Da.data(), t1.data());
-                hfx_la::mm(n, t1.data(), Fa.data(), ea.data());
-                for (size_t x = 0; x < ea.size(); ++x) ea[x] = t2[x] - ea[x];
-                hfx_la::mm(n, Fb.data(), Db.data(), t1.data());
-                hfx_la::mm(n, t1.data(), S.data(), t2.data());
-                hfx_la::mm(n, S.data(), Db.data(), t1.data());
-                hfx_la::mm(n, t1.data(), Fb.data(), eb.data());
-                for (size_t x = 0; x < eb.size(); ++x) eb[x] = t2[x] - eb[x] + ea[x];
-                diis->update(Fa, Fb, eb);
-                diis->extrapolate(Fa, Fb);
-            }
-            diagonalize_and_update();
-            dE = std::abs(e_elec - last);
-            {
-                double s = 0.0;
-                for (size_t x = 0; x < Dt.size(); ++x) {
-                    const double d = Dt[x] - Dt_prev[x];
-                    s += d * d;
-                }
-                dD = std::sqrt(s);
-            }
-            diis_err = diis ? diis->err_norm() : 0.0;
-            const int slot = trace_base + iteration;
-            if (res && res->trace && slot < res->trace_cap) {
-                res->trace[4 * slot + 0] = e_elec;
-                res->trace[4 * slot + 1] = dE;
-                res->trace[4 * slot + 2] = dD;
-                res->trace[4 * slot + 3] = diis_err;
-            }
-            if (dE < o.energy_tol && dD < o.density_tol && (!diis || diis_err < o.diis_error_tol)) {
+            if (after_build(diis, last, res, trace_base + iteration)) {
                 conv = true;
                 it_done = iteration + 1;
                 return HFX_OK;
             }
         }
         it_done = o.max_iter;
+        return HFX_OK;
+    }
+
+    void remove_level_shift() {  // SCF.cpp:93-107
+        if (ls_a)
+            for (int i = na; i < n; ++i) eps_a[i] -= o.level_shift;
+        if (ls_b)
+            for (int i = nb; i < n; ++i) eps_b[i] -= o.level_shift;
+    }
+
+    void write_result(hfx_scf_result* res) const {
+        res->e_electronic = e_elec;
+        res->e_nuclear = e_nuc;
+        res->e_total = e_elec + e_nuc;
+        res->last_dE = dE;
+        res->last_dD = dD;
+        res->last_diis_err = diis_err;
+        res->time_jk_s = t_jk;
+        res->time_diag_s = t_diag;
+        res->n_jk_builds = n_builds;
+        res->n_quartets_last = n_quart_last;
+        auto put = [&](double* dst, const Mat& src) {
+            if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
+        };
+        put(res->F_alpha, Fa);
+        put(res->D_alpha, Da);
+        put(res->C_alpha, Ca);
+        put(res->F_beta, Fb);
+        put(res->D_beta, Db);
+        put(res->C_beta, Cb);
+        put(res->h_core, h);
+        put(res->S, S);
+        if (res->eps_alpha) std::memcpy(res->eps_alpha, eps_a.data(), n * sizeof(double));
+        if (res->eps_beta) std::memcpy(res->eps_beta, eps_b.data(), n * sizeof(double));
+    }
+
+    // Electron counts (SCF.cpp:140-142, Molecule.cpp:67-72) and the nuclear repulsion
+    // (Molecule.cpp:75-89).
+    int setup_molecule(int n_atoms, const double* Z, const double* xyz, int charge, int mult) {
+        long nel = -charge;
+        for (int i = 0; i < n_atoms; ++i) nel += (long)std::lround(Z[i]);
+        if (mult < 1 || nel < 0 || ((nel + mult - 1) % 2) != 0 || nel + 1 < mult) return HFX_ERR_INVALID;
+        na = (int)((nel + mult - 1) / 2);
+        nb = (int)((nel - mult + 1) / 2);
+        if (na > n) return HFX_ERR_INVALID;
+        uhf = o.unrestricted < 0 ? (mult != 1) : (o.unrestricted != 0);
+        precision = o.precision;
+        double en = 0.0;
+        for (int i = 0; i < n_atoms; ++i)
+            for (int j = i + 1; j < n_atoms; ++j) {
+                const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                             dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+                en += (Z[i] * Z[j]) / std::sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        e_nuc = en;
         return HFX_OK;
     }
 };
@@ -380,25 +447,9 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
     if (nmol != 1) return HFX_ERR_INVALID;
     s.n = (int)N;
     s.nS = (int)nS;
-    // Electron counts (SCF.cpp:140-142, Molecule.cpp:67-72).
-    long nel = -charge;
-    for (int i = 0; i < n_atoms; ++i) nel += (long)std::lround(Z[i]);
-    if (mult < 1 || nel < 0 || ((nel + mult - 1) % 2) != 0 || nel + 1 < mult) return HFX_ERR_INVALID;
-    s.na = (int)((nel + mult - 1) / 2);
-    s.nb = (int)((nel - mult + 1) / 2);
-    if (s.na > s.n) return HFX_ERR_INVALID;
-    s.uhf = s.o.unrestricted < 0 ? (mult != 1) : (s.o.unrestricted != 0);
-    s.precision = s.o.precision;
+    if ((rc = s.setup_molecule(n_atoms, Z, xyz, charge, mult))) return rc;
 
     // SCF::initialize (SCF.cpp:138-162).
-    double en = 0.0;
-    for (int i = 0; i < n_atoms; ++i)
-        for (int j = i + 1; j < n_atoms; ++j) {
-            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
-                         dz = xyz[3 * i + 2] - xyz[3 * j + 2];
-            en += (Z[i] * Z[j]) / std::sqrt(dx * dx + dy * dy + dz * dz);
-        }
-    s.e_nuc = en;
     Mat T = s.mat(), V = s.mat();
     s.S = s.mat();
     s.Q.assign((size_t)nS * nS, 0.0);
@@ -429,38 +480,11 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
         if (rc) return rc;
         total_its += its2;
     }
-    // Remove the level shift from the virtual eigenvalues (SCF.cpp:93-107).
-    if (s.ls_a)
-        for (int i = s.na; i < s.n; ++i) s.eps_a[i] -= s.o.level_shift;
-    if (s.ls_b)
-        for (int i = s.nb; i < s.n; ++i) s.eps_b[i] -= s.o.level_shift;
-
-    res->e_electronic = s.e_elec;
-    res->e_nuclear = s.e_nuc;
-    res->e_total = s.e_elec + s.e_nuc;
+    s.remove_level_shift();
+    s.write_result(res);
     res->iterations = total_its;
     res->converged = conv ? 1 : 0;
-    res->last_dE = s.dE;
-    res->last_dD = s.dD;
-    res->last_diis_err = s.diis_err;
     res->time_init_s = secs(t0, t_init);
-    res->time_jk_s = s.t_jk;
-    res->time_diag_s = s.t_diag;
-    res->n_jk_builds = s.n_builds;
-    res->n_quartets_last = s.n_quart_last;
-    auto put = [&](double* dst, const Mat& src) {
-        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
-    };
-    put(res->F_alpha, s.Fa);
-    put(res->D_alpha, s.Da);
-    put(res->C_alpha, s.Ca);
-    put(res->F_beta, s.Fb);
-    put(res->D_beta, s.Db);
-    put(res->C_beta, s.Cb);
-    put(res->h_core, s.h);
-    put(res->S, s.S);
-    if (res->eps_alpha) std::memcpy(res->eps_alpha, s.eps_a.data(), s.n * sizeof(double));
-    if (res->eps_beta) std::memcpy(res->eps_beta, s.eps_b.data(), s.n * sizeof(double));
     res->time_total_s = secs(t0, clk::now());
     return HFX_OK;
 }
@@ -478,6 +502,193 @@ int hfx_scf_run(const hfx_basis* basis, int32_t n_atoms, const double* Z, const 
         res->time_init_s += t_create;
         res->time_total_s += t_create;
     }
+    return rc;
+}
+
+// Many molecules through one batched engine.  Each molecule runs the cycles of SCF::run
+// (SCF.cpp:51-114) with its own DIIS, damping and level-shift state, but every iteration's
+// Fock builds go out as ONE hfx_jk over the packed density increments of all molecules.  A
+// molecule that has converged contributes a zero increment, which the density screening
+// removes before any integral work, and its Fock matrix is left alone.
+int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_aos_mol, int32_t n_atoms,
+                               const double* Z, const double* xyz, const int32_t* atom_mol,
+                               const int32_t* charge, const int32_t* mult, const hfx_scf_options* opt,
+                               hfx_scf_result* res) {
+    if (!e || !res || n_mol <= 0 || !n_aos_mol || n_atoms <= 0 || !Z || !xyz || !atom_mol)
+        return HFX_ERR_INVALID;
+    const auto t0 = clk::now();
+    int64_t N = 0, nS = 0, nm = 0, elems = 0;
+    int rc = hfx_sizes(e, &N, &nS, &nm, &elems);
+    if (rc) return rc;
+    if (nm != n_mol) return HFX_ERR_INVALID;
+    hfx_scf_options o;
+    if (opt)
+        o = *opt;
+    else
+        hfx_scf_default_options(&o);
+
+    // Per-molecule atoms (contiguous runs of atom_mol) and packed block offsets.
+    std::vector<int> atom_begin(n_mol + 1, 0);
+    for (int i = 0; i < n_atoms; ++i) {
+        const int m = atom_mol[i];
+        if (m < 0 || m >= n_mol || (i > 0 && m < atom_mol[i - 1])) return HFX_ERR_INVALID;
+        atom_begin[m + 1] = i + 1;
+    }
+    for (int m = 0; m < n_mol; ++m)
+        if (atom_begin[m + 1] < atom_begin[m]) atom_begin[m + 1] = atom_begin[m];
+    std::vector<size_t> off(n_mol + 1, 0);
+    for (int m = 0; m < n_mol; ++m) off[m + 1] = off[m] + (size_t)n_aos_mol[m] * n_aos_mol[m];
+    if ((int64_t)off[n_mol] != elems) return HFX_ERR_INVALID;
+
+    std::vector<Scf> mol(n_mol);
+    for (int m = 0; m < n_mol; ++m) {
+        Scf& s = mol[m];
+        s.e = e;
+        s.o = o;
+        s.n = n_aos_mol[m];
+        const int a0 = atom_begin[m], na = atom_begin[m + 1] - a0;
+        if (na <= 0) return HFX_ERR_INVALID;
+        if ((rc = s.setup_molecule(na, Z + a0, xyz + 3 * a0, charge ? charge[m] : 0, mult ? mult[m] : 1)))
+            return rc;
+    }
+    bool any_uhf = false;
+    for (const Scf& s : mol) any_uhf |= s.uhf;
+
+    // SCF::initialize (SCF.cpp:138-162) for all molecules from the packed one-electron blocks.
+    {
+        Mat T(elems), V(elems), S(elems);
+        if ((rc = hfx_kinetic(e, T.data())) || (rc = hfx_nuclear(e, n_atoms, Z, xyz, atom_mol, V.data())) ||
+            (rc = hfx_overlap(e, S.data())))
+            return rc;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int m = 0; m < n_mol; ++m) {
+            Scf& s = mol[m];
+            s.S.assign(S.begin() + off[m], S.begin() + off[m + 1]);
+            s.h.resize(off[m + 1] - off[m]);
+            for (size_t x = 0; x < s.h.size(); ++x) s.h[x] = T[off[m] + x] + V[off[m] + x];
+            s.X = inv_sqrt(s.n, s.S);
+            s.initial_guess();
+        }
+    }
+    const auto t_init = clk::now();
+
+    Mat dDt(elems), dDa(elems), dDb(any_uhf ? elems : 0), J(elems), Ka(elems), Kb(any_uhf ? elems : 0);
+    std::vector<int> its(n_mol, 0), its_mixed(n_mol, 0), done(n_mol, 0), conv(n_mol, 0);
+    std::vector<double> e_mixed(n_mol, 0.0);
+    double t_jk = 0.0;
+    int64_t n_builds = 0, n_quart_last = 0;
+
+    auto stage = [&](int precision, bool first_full) -> int {
+        std::vector<Diis> diis(n_mol, Diis(o.diis_size));
+        std::vector<int> active(n_mol, 1);
+        for (int m = 0; m < n_mol; ++m) {
+            mol[m].precision = precision;
+            mol[m].iteration = 0;
+        }
+        for (int it = 0; it < o.max_iter; ++it) {
+            int n_active = 0;
+            for (int m = 0; m < n_mol; ++m) n_active += active[m];
+            if (!n_active) break;
+            std::fill(dDt.begin(), dDt.end(), 0.0);
+            std::fill(dDa.begin(), dDa.end(), 0.0);
+            std::fill(dDb.begin(), dDb.end(), 0.0);
+            std::vector<double> last(n_mol);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int m = 0; m < n_mol; ++m) {
+                if (!active[m]) continue;
+                Scf& s = mol[m];
+                last[m] = s.e_elec;
+                double* db = any_uhf ? dDb.data() + off[m] : nullptr;
+                s.deltas(o.full_builds || (first_full && it == 0), dDt.data() + off[m], dDa.data() + off[m],
+                         s.uhf ? db : nullptr);
+                // An RHF molecule in a batch with UHF molecules: beta increment = alpha increment.
+                if (db && !s.uhf) std::copy(dDa.begin() + off[m], dDa.begin() + off[m + 1], dDb.begin() + off[m]);
+            }
+            hfx_jk_args a{};
+            a.dD_tot = dDt.data();
+            a.dD_alpha = dDa.data();
+            a.dD_beta = any_uhf ? dDb.data() : nullptr;
+            a.Q = nullptr;
+            a.schwarz_thresh = o.schwarz_thresh;
+            a.density_thresh = o.density_thresh;
+            a.precision = precision;
+            a.fp32_cut = o.fp32_cut;
+            a.flags = o.jk_flags;
+            hfx_jk_stats st{};
+            const auto tj = clk::now();
+            if ((rc = hfx_jk(e, &a, J.data(), Ka.data(), any_uhf ? Kb.data() : nullptr, &st))) return rc;
+            t_jk += secs(tj, clk::now());
+            n_builds++;
+            n_quart_last = st.n_quartets;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int m = 0; m < n_mol; ++m) {
+                if (!active[m]) continue;
+                Scf& s = mol[m];
+                s.iteration = it;
+                s.apply_jk(J.data() + off[m], Ka.data() + off[m], s.uhf ? Kb.data() + off[m] : nullptr);
+                const int slot = its[m] + it;
+                if (s.after_build(o.use_diis ? &diis[m] : nullptr, last[m], res ? &res[m] : nullptr, slot)) {
+                    active[m] = 0;
+                    conv[m] = 1;
+                    done[m] = it + 1;
+                }
+            }
+        }
+        for (int m = 0; m < n_mol; ++m) {
+            if (active[m]) {
+                conv[m] = 0;
+                done[m] = o.max_iter;
+            }
+        }
+        return HFX_OK;
+    };
+
+    if ((rc = stage(o.precision, false))) return rc;
+    for (int m = 0; m < n_mol; ++m) {
+        its[m] = done[m];
+        its_mixed[m] = done[m];
+        e_mixed[m] = mol[m].e_elec + mol[m].e_nuc;
+    }
+    if (o.precision == HFX_MIXED && o.final_fp64) {
+        if ((rc = stage(HFX_FP64, true))) return rc;
+        for (int m = 0; m < n_mol; ++m) its[m] += done[m];
+    }
+    const double t_total = secs(t0, clk::now());
+    for (int m = 0; m < n_mol; ++m) {
+        Scf& s = mol[m];
+        s.remove_level_shift();
+        s.t_jk = t_jk;
+        s.n_builds = n_builds;
+        s.n_quart_last = n_quart_last;
+        s.write_result(&res[m]);
+        res[m].iterations = its[m];
+        res[m].converged = conv[m];
+        res[m].e_total_mixed = e_mixed[m];
+        res[m].iterations_mixed = its_mixed[m];
+        res[m].time_init_s = secs(t0, t_init);
+        res[m].time_total_s = t_total;
+    }
+    return HFX_OK;
+}
+
+int hfx_scf_run_batched(int32_t n_mol, const hfx_basis* bases, int32_t n_atoms, const double* Z,
+                        const double* xyz, const int32_t* atom_mol, const int32_t* charge, const int32_t* mult,
+                        const hfx_scf_options* opt, int device, hfx_scf_result* res) {
+    if (n_mol <= 0 || !bases || !res) return HFX_ERR_INVALID;
+    hfx_engine* e = nullptr;
+    const auto t0 = clk::now();
+    int rc = hfx_create_batched(n_mol, bases, device, &e);
+    if (rc) return rc;
+    const double t_create = secs(t0, clk::now());
+    std::vector<int32_t> n_aos(n_mol);
+    for (int m = 0; m < n_mol; ++m) n_aos[m] = bases[m].n_aos;
+    rc = hfx_scf_run_batched_engine(e, n_mol, n_aos.data(), n_atoms, Z, xyz, atom_mol, charge, mult, opt, res);
+    hfx_destroy(e);
+    if (rc == HFX_OK)
+        for (int m = 0; m < n_mol; ++m) {
+            res[m].time_init_s += t_create;
+            res[m].time_total_s += t_create;
+        }
     return rc;
 }
 

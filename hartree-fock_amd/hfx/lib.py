@@ -56,6 +56,16 @@ class HfxJkStats(C.Structure):
                     class_ms32=list(self.class_ms32))
 
 
+class HfxGradArgs(C.Structure):
+    _fields_ = [("P", _f64p), ("W", _f64p), ("n_atoms", C.c_int32), ("Z", _f64p), ("xyz", _f64p),
+                ("shell_atom", _i32p), ("schwarz_thresh", C.c_double), ("density_thresh", C.c_double),
+                ("precision", C.c_int32), ("fp32_cut", C.c_double), ("flags", C.c_int32)]
+
+
+class HfxGradStats(C.Structure):
+    _fields_ = [("n_quartets", C.c_int64), ("n_fp32", C.c_int64), ("device_ms", C.c_double)]
+
+
 class HfxScfOptions(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("energy_tol", C.c_double), ("density_tol", C.c_double),
                 ("use_diis", C.c_int32), ("diis_size", C.c_int32), ("diis_error_tol", C.c_double),
@@ -81,7 +91,8 @@ class HfxScfResult(C.Structure):
 
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
                "hfx_nuclear", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk", "hfx_last_error",
-               "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine"]
+               "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine", "hfx_scf_run_batched",
+               "hfx_scf_run_batched_engine", "hfx_rhf_gradient"]
 
 
 def _p(a, t=_f64p):
@@ -111,12 +122,17 @@ class Library:
         d.hfx_nuclear.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _i32p, _f64p]
         d.hfx_eri_quartets.argtypes = [C.c_void_p, C.c_int64, _i32p, C.POINTER(C.c_int64), _f64p]
         d.hfx_jk.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.POINTER(HfxJkStats)]
+        d.hfx_rhf_gradient.argtypes = [C.c_void_p, C.POINTER(HfxGradArgs), _f64p, _f64p, C.POINTER(HfxGradStats)]
         d.hfx_scf_default_options.argtypes = [C.POINTER(HfxScfOptions)]
         d.hfx_scf_default_options.restype = None
         d.hfx_scf_run.argtypes = [C.POINTER(HfxBasis), C.c_int32, _f64p, _f64p, C.c_int32, C.c_int32,
                                   C.POINTER(HfxScfOptions), C.c_int, C.POINTER(HfxScfResult)]
         d.hfx_scf_run_engine.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, C.c_int32, C.c_int32,
                                          C.POINTER(HfxScfOptions), C.POINTER(HfxScfResult)]
+        d.hfx_scf_run_batched.argtypes = [C.c_int32, C.POINTER(HfxBasis), C.c_int32, _f64p, _f64p, _i32p, _i32p,
+                                          _i32p, C.POINTER(HfxScfOptions), C.c_int, C.POINTER(HfxScfResult)]
+        d.hfx_scf_run_batched_engine.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int32, _f64p, _f64p, _i32p,
+                                                 _i32p, _i32p, C.POINTER(HfxScfOptions), C.POINTER(HfxScfResult)]
 
     def check(self, rc, what):
         if rc != HFX_OK:
@@ -261,6 +277,36 @@ class Engine:
         return self.unpack(J), self.unpack(Ka), (None if Kb is None else self.unpack(Kb)), st.as_dict()
 
 
+def rhf_gradient(engine, basis, P, W, schwarz_thresh=1e-10, density_thresh=1e-10, precision=HFX_FP64,
+                 fp32_cut=0.0, flags=0):
+    """RHF nuclear gradient (n_atoms, 3) from the total density P and the energy-weighted
+    density W.  Returns (grad, parts, stats) with parts[k] (n_atoms, 3) for k = nuclear
+    repulsion, -W.dS, P.d(T+V), two-electron."""
+    nat = len(basis.Z)
+    Z = np.ascontiguousarray(np.asarray(basis.Z, np.float64))
+    xyz = np.ascontiguousarray(basis.xyz.reshape(-1))
+    sat = np.ascontiguousarray([sh.atom for sh in basis.shells], np.int32)
+    Pf = np.ascontiguousarray(np.asarray(P, np.float64).reshape(-1, order="F"))
+    Wf = np.ascontiguousarray(np.asarray(W, np.float64).reshape(-1, order="F"))
+    a = HfxGradArgs()
+    a.P, a.W, a.n_atoms, a.Z, a.xyz, a.shell_atom = _p(Pf), _p(Wf), nat, _p(Z), _p(xyz), _p(sat, _i32p)
+    a.schwarz_thresh, a.density_thresh = schwarz_thresh, density_thresh
+    a.precision, a.fp32_cut, a.flags = precision, fp32_cut, flags
+    g = np.zeros(3 * nat)
+    parts = np.zeros(4 * 3 * nat)
+    st = HfxGradStats()
+    engine.lib.check(engine.lib.dll.hfx_rhf_gradient(engine.h, C.byref(a), _p(g), _p(parts), C.byref(st)),
+                     "hfx_rhf_gradient")
+    return g.reshape(nat, 3), parts.reshape(4, nat, 3), dict(n_quartets=st.n_quartets, n_fp32=st.n_fp32,
+                                                            device_ms=st.device_ms)
+
+
+def energy_weighted_density(C, eps, n_occ):
+    """W = 2 sum_{i < n_occ} eps_i C_i C_i^T (RHF)."""
+    Co = C[:, :n_occ]
+    return 2.0 * (Co * eps[:n_occ]) @ Co.T
+
+
 def jk_device(engine, dD_tot, dD_alpha, J, Ka, dD_beta=None, Kb=None, Q=None, schwarz_thresh=1e-10,
               density_thresh=1e-10, precision=HFX_FP64, fp32_cut=0.0, flags=0):
     """One build on device-resident buffers (hfx.device.DeviceArray); returns the stats dict."""
@@ -305,6 +351,34 @@ def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, ou
             continue
         out[k] = v.reshape(n, n, order="F") if v.size == n * n else v
     return out
+
+
+def scf_run_batched(library, bases, charges=None, multiplicities=None, options=None, device=0, engine=None):
+    """Run the batched SCF driver over a list of Basis objects (one per molecule).
+
+    Returns a list of per-molecule dicts (energies, iterations, convergence, timings)."""
+    o = options if options is not None else library.default_options()
+    n_mol = len(bases)
+    res = (HfxScfResult * n_mol)()
+    Z = np.ascontiguousarray(np.concatenate([np.asarray(b.Z, np.float64) for b in bases]))
+    xyz = np.ascontiguousarray(np.concatenate([b.xyz.reshape(-1) for b in bases]))
+    atom_mol = np.ascontiguousarray(np.concatenate([np.full(len(b.Z), m, np.int32) for m, b in enumerate(bases)]))
+    ch = None if charges is None else np.ascontiguousarray(charges, np.int32)
+    mu = None if multiplicities is None else np.ascontiguousarray(multiplicities, np.int32)
+    if engine is None:
+        holders = [_BasisHolder(b) for b in bases]
+        arr = (HfxBasis * n_mol)(*[x.struct for x in holders])
+        rc = library.dll.hfx_scf_run_batched(n_mol, arr, len(Z), _p(Z), _p(xyz), _p(atom_mol, _i32p),
+                                             _p(ch, _i32p), _p(mu, _i32p), C.byref(o), device, res)
+        library.check(rc, "hfx_scf_run_batched")
+    else:
+        n_aos = np.ascontiguousarray([b.n_aos for b in bases], np.int32)
+        rc = library.dll.hfx_scf_run_batched_engine(engine.h, n_mol, _p(n_aos, _i32p), len(Z), _p(Z), _p(xyz),
+                                                    _p(atom_mol, _i32p), _p(ch, _i32p), _p(mu, _i32p),
+                                                    C.byref(o), res)
+        library.check(rc, "hfx_scf_run_batched_engine")
+    keys = [k for k, _ in HfxScfResult._fields_ if not k.startswith(("F_", "D_", "C_", "eps_", "h_", "S", "trace"))]
+    return [{k: getattr(r, k) for k in keys} for r in res]
 
 
 _LIB = None

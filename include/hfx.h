@@ -146,6 +146,37 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb,
            hfx_jk_stats* stats);
 
+/* RHF nuclear gradient inputs.  No reference counterpart: the reference computes energies
+ * only (src/SCF.cpp:23-136); this is the analytic derivative of that energy,
+ *   dE/dX = sum P dh/dX - sum W dS/dX + 1/2 sum Gamma d(mn|ls)/dX + dVnn/dX,
+ *   Gamma_mnls = P_mn P_ls - 1/4 (P_ml P_ns + P_ms P_nl),
+ * with P the total density and W = 2 sum_occ eps_i C_i C_i^T. */
+typedef struct hfx_grad_args {
+    const double* P;           /* N x N total density                                      */
+    const double* W;           /* N x N energy-weighted density                            */
+    int32_t n_atoms;
+    const double* Z;           /* nuclear charges                                          */
+    const double* xyz;         /* 3 x n_atoms positions (Bohr); shells must sit on atoms    */
+    const int32_t* shell_atom; /* atom of every shell, or NULL to match shell centres       */
+    double schwarz_thresh;     /* quartets with Q(ab)Q(cd) < tau_S are skipped               */
+    double density_thresh;     /* and those failing the hfx_jk density test on (P, P/2)      */
+    int32_t precision;         /* HFX_FP64, or HFX_MIXED with fp32_cut as in hfx_jk          */
+    double fp32_cut;
+    int32_t flags;             /* HFX_JK_DEVICE_PTRS: P and W are device pointers            */
+} hfx_grad_args;
+
+typedef struct hfx_grad_stats {
+    int64_t n_quartets;        /* shell quartets differentiated                            */
+    int64_t n_fp32;
+    double device_ms;          /* device time of the two-electron part                     */
+} hfx_grad_stats;
+
+/* grad[3*n_atoms] (x,y,z per atom).  parts, when not NULL, receives 4 x 3*n_atoms:
+ * nuclear repulsion, -W.dS, P.d(T+V) (basis and operator derivatives), two-electron.
+ * Single-molecule engines only.  stats may be NULL. */
+int hfx_rhf_gradient(hfx_engine* e, const hfx_grad_args* args, double* grad, double* parts,
+                     hfx_grad_stats* stats);
+
 /* Thread-local description of the last error on this thread. */
 const char* hfx_last_error(void);
 

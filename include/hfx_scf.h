@@ -82,6 +82,24 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
                        int32_t charge, int32_t multiplicity, const hfx_scf_options* opt,
                        hfx_scf_result* res);
 
+/* Many independent molecules through one batched engine (hfx_create_batched): each molecule
+ * runs SCF::run (src/SCF.cpp:23-136) with its own DIIS/damping/level-shift state, and every
+ * iteration's Fock builds of all molecules still iterating are ONE hfx_jk call.  Atoms of all
+ * molecules are concatenated in molecule order, atom_mol[i] names the molecule of atom i;
+ * charge and multiplicity are per molecule (NULL: neutral singlets); res holds n_mol results.
+ * time_jk_s and n_jk_builds count the shared batched builds.  No reference counterpart (the
+ * reference runs one molecule per process). */
+int hfx_scf_run_batched(int32_t n_mol, const hfx_basis* bases, int32_t n_atoms, const double* Z,
+                        const double* xyz, const int32_t* atom_mol, const int32_t* charge,
+                        const int32_t* multiplicity, const hfx_scf_options* opt, int device,
+                        hfx_scf_result* res);
+
+/* The same over a batched engine the caller created; n_aos_mol[m] = AOs of molecule m. */
+int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_aos_mol, int32_t n_atoms,
+                               const double* Z, const double* xyz, const int32_t* atom_mol,
+                               const int32_t* charge, const int32_t* multiplicity,
+                               const hfx_scf_options* opt, hfx_scf_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -761,6 +761,330 @@ void direct_jk(Mol& m, const double* dDt, const double* dDa, const double* dDb, 
     }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// RHF nuclear gradient.  No reference counterpart (the reference computes energies only):
+// the analytic derivative of the energy that the functions above define,
+//   dE/dX = sum P dh/dX - sum W dS/dX + 1/2 sum Gamma d(mn|ls)/dX + dVnn/dX.
+// A Cartesian Gaussian's centre derivative is d/dA_x g_i = 2a g_{i+1} - i g_{i-1}; applied
+// to the 1-D Hermite coefficients it gives 2a E^{i+1,j}_t - i E^{i-1,j}_t (and likewise on
+// the second centre with b and j).  The operator derivative of the nuclear attraction is
+// dR_tuv/dC_x = -R_{t+1,u,v}.  Every centre is differentiated explicitly (no translational
+// invariance is used), so sum over atoms of the gradient = 0 is a check, not a construction.
+// Pinned by central finite differences of the SCF energy (tests/).
+// ---------------------------------------------------------------------------------------
+
+// 1-D coefficient of the base (w = 0), first-centre (w = 1) or second-centre (w = 2)
+// derivative; E holds extents (l1+1, l2+1) so every shifted index exists.
+double e_der(const EBuf& E, int i, int j, int t, int w, double a, double b) {
+    auto at = [&](int ii, int jj) { return (ii < 0 || jj < 0 || t > ii + jj) ? 0.0 : E.at(ii, jj, t); };
+    if (w == 0) return at(i, j);
+    if (w == 1) return 2.0 * a * at(i + 1, j) - (double)i * at(i - 1, j);
+    return 2.0 * b * at(i, j + 1) - (double)j * at(i, j - 1);
+}
+
+// (ab|cd) from per-dimension Hermite vectors: e[d][t] for the bra, f[d][t] for the ket.
+double hermite_contract(const double* const e[3], const int ne[3], const double* const f[3], const int nf[3],
+                        RBuf& R) {
+    double acc = 0.0;
+    for (int t = 0; t < ne[0]; ++t)
+        for (int u = 0; u < ne[1]; ++u)
+            for (int v = 0; v < ne[2]; ++v) {
+                const double ebra = e[0][t] * e[1][u] * e[2][v];
+                if (ebra == 0.0) continue;
+                double s = 0.0;
+                for (int tt = 0; tt < nf[0]; ++tt)
+                    for (int uu = 0; uu < nf[1]; ++uu)
+                        for (int vv = 0; vv < nf[2]; ++vv) {
+                            const double sg = ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+                            s += sg * f[0][tt] * f[1][uu] * f[2][vv] * R.at(t + tt, u + uu, v + vv, 0);
+                        }
+                acc += ebra * s;
+            }
+    return acc;
+}
+
+// g_out[4][3]: derivative of sum_{block} Gam * (ab|cd) w.r.t. the centres of a, b, c, d.
+void eri_quartet_grad(const Mol& m, int sa, int sb, int sc, int sd, const double* Gam, double g_out[4][3]) {
+    const int s[4] = {sa, sb, sc, sd};
+    int nao[4], l[4], ao[4];
+    for (int k = 0; k < 4; ++k) {
+        nao[k] = m.nao[s[k]];
+        l[k] = m.l[s[k]];
+        ao[k] = m.aoff[s[k]];
+    }
+    const unsigned L = l[0] + l[1] + l[2] + l[3] + 1;
+    RBuf R(L);
+    std::vector<double> F(L + 1);
+    EBuf Eb[3] = {EBuf(l[0] + 1, l[1] + 1), EBuf(l[0] + 1, l[1] + 1), EBuf(l[0] + 1, l[1] + 1)};
+    EBuf Ek[3] = {EBuf(l[2] + 1, l[3] + 1), EBuf(l[2] + 1, l[3] + 1), EBuf(l[2] + 1, l[3] + 1)};
+    const double* ctr[4];
+    for (int k = 0; k < 4; ++k) ctr[k] = &m.ctr[3 * s[k]];
+    for (int k = 0; k < 4; ++k)
+        for (int x = 0; x < 3; ++x) g_out[k][x] = 0.0;
+    const int nB = nao[1], nC = nao[2], nD = nao[3];
+    for (int pa = 0; pa < m.nprim[sa]; ++pa)
+        for (int pb = 0; pb < m.nprim[sb]; ++pb) {
+            const double a = m.alpha[m.poff[sa] + pa], b = m.alpha[m.poff[sb] + pb];
+            const PairData ab = pair_data(a, ctr[0], b, ctr[1]);
+            for (int x = 0; x < 3; ++x) hermite(l[0] + 1, l[1] + 1, ab.p, ab.PA[x], ab.PB[x], Eb[x]);
+            for (int pc = 0; pc < m.nprim[sc]; ++pc)
+                for (int pd = 0; pd < m.nprim[sd]; ++pd) {
+                    const double c = m.alpha[m.poff[sc] + pc], d = m.alpha[m.poff[sd] + pd];
+                    const PairData cd = pair_data(c, ctr[2], d, ctr[3]);
+                    for (int x = 0; x < 3; ++x) hermite(l[2] + 1, l[3] + 1, cd.p, cd.PA[x], cd.PB[x], Ek[x]);
+                    const double pref = (2.0 * std::pow(M_PI, 2.5) / (ab.p * cd.p * std::sqrt(ab.p + cd.p))) *
+                                        ab.K * cd.K;
+                    const double delta = (ab.p * cd.p) / (ab.p + cd.p);
+                    const double PQ[3] = {ab.P[0] - cd.P[0], ab.P[1] - cd.P[1], ab.P[2] - cd.P[2]};
+                    boys(L, delta * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), F.data());
+                    hermite_r(delta, PQ, F.data(), R);
+                    const double cprod = m.coeff[m.poff[sa] + pa] * m.coeff[m.poff[sb] + pb] *
+                                         m.coeff[m.poff[sc] + pc] * m.coeff[m.poff[sd] + pd] * pref;
+                    const double* na = &m.norm[m.noff[sa] + pa * nao[0]];
+                    const double* nb = &m.norm[m.noff[sb] + pb * nao[1]];
+                    const double* nc = &m.norm[m.noff[sc] + pc * nao[2]];
+                    const double* nd = &m.norm[m.noff[sd] + pd * nao[3]];
+                    // vectors: [w][dim][t], w = 0 base, 1 first centre, 2 second centre
+                    double eb[3][3][8], fk[3][3][8];
+                    for (int A = 0; A < nao[0]; ++A)
+                        for (int B = 0; B < nB; ++B) {
+                            const int* c1 = &m.cart[3 * (ao[0] + A)];
+                            const int* c2 = &m.cart[3 * (ao[1] + B)];
+                            int nbra[3];
+                            for (int x = 0; x < 3; ++x) {
+                                nbra[x] = c1[x] + c2[x] + 2;
+                                for (int w = 0; w < 3; ++w)
+                                    for (int t = 0; t < nbra[x]; ++t) eb[w][x][t] = e_der(Eb[x], c1[x], c2[x], t, w, a, b);
+                            }
+                            for (int C = 0; C < nC; ++C)
+                                for (int D = 0; D < nD; ++D) {
+                                    const double gm =
+                                        Gam[((size_t)A * nB + B) * nC * nD + (size_t)C * nD + D] * na[A] * nb[B] *
+                                        nc[C] * nd[D] * cprod;
+                                    if (gm == 0.0) continue;
+                                    const int* c3 = &m.cart[3 * (ao[2] + C)];
+                                    const int* c4 = &m.cart[3 * (ao[3] + D)];
+                                    int nket[3];
+                                    for (int x = 0; x < 3; ++x) {
+                                        nket[x] = c3[x] + c4[x] + 2;
+                                        for (int w = 0; w < 3; ++w)
+                                            for (int t = 0; t < nket[x]; ++t)
+                                                fk[w][x][t] = e_der(Ek[x], c3[x], c4[x], t, w, c, d);
+                                    }
+                                    for (int x = 0; x < 3; ++x) {
+                                        // bra centres: dimension x takes the derivative vector
+                                        for (int w = 1; w <= 2; ++w) {
+                                            const double* e3[3] = {eb[0][0], eb[0][1], eb[0][2]};
+                                            e3[x] = eb[w][x];
+                                            const double* f3[3] = {fk[0][0], fk[0][1], fk[0][2]};
+                                            g_out[w - 1][x] += gm * hermite_contract(e3, nbra, f3, nket, R);
+                                        }
+                                        for (int w = 1; w <= 2; ++w) {
+                                            const double* e3[3] = {eb[0][0], eb[0][1], eb[0][2]};
+                                            const double* f3[3] = {fk[0][0], fk[0][1], fk[0][2]};
+                                            f3[x] = fk[w][x];
+                                            g_out[1 + w][x] += gm * hermite_contract(e3, nbra, f3, nket, R);
+                                        }
+                                    }
+                                }
+                        }
+                }
+        }
+}
+
+// Two-electron gradient: unique shell quartets (a>=b, c>=d, ab>=cd) with the J/K build's
+// Schwarz and density tests on (P, P/2); every quartet counts deg = 2^(#distinct swaps)
+// times, and the energy carries 1/2 sum Gamma (mn|ls).
+void grad_eri(Mol& m, const double* P, const int* shell_atom, double tauS, double tauD, double* g,
+              int64_t* n_quartets) {
+    const int nS = m.nS, N = m.N;
+    schwarz(m);
+    const double* Q = m.Q.data();
+    std::vector<double> mt((size_t)nS * nS, 0.0);
+    for (int s1 = 0; s1 < nS; ++s1)
+        for (int s2 = 0; s2 < nS; ++s2) {
+            double v = 0;
+            for (int a = m.aoff[s1]; a < m.aoff[s1] + m.nao[s1]; ++a)
+                for (int b = m.aoff[s2]; b < m.aoff[s2] + m.nao[s2]; ++b) v = std::max(v, std::abs(P[a + (size_t)b * N]));
+            mt[s1 + (size_t)s2 * nS] = v;
+        }
+    auto MT = [&](int r, int c) { return mt[r + (size_t)c * nS]; };
+    auto QQ = [&](int r, int c) { return Q[r + (size_t)c * nS]; };
+    const int nat = 1 + *std::max_element(shell_atom, shell_atom + nS);
+    std::vector<double> gt(3 * (size_t)nat, 0.0);
+    int64_t nq = 0;
+#pragma omp parallel
+    {
+        std::vector<double> gp(3 * (size_t)nat, 0.0), Gam;
+        int64_t lq = 0;
+#pragma omp for collapse(4) schedule(dynamic, 16) nowait
+        for (int a = 0; a < nS; ++a)
+            for (int b = 0; b < nS; ++b)
+                for (int c = 0; c < nS; ++c)
+                    for (int d = 0; d < nS; ++d) {
+                        if (b > a || d > c || (a * (a + 1) / 2 + b) < (c * (c + 1) / 2 + d)) continue;
+                        if (QQ(a, b) * QQ(c, d) < tauS) continue;
+                        // density test of the J/K build with dD_tot = P, dD_alpha = P/2
+                        const double mx = std::max({2 * MT(a, b), 2 * MT(c, d), 0.5 * MT(a, c), 0.5 * MT(a, d),
+                                                    0.5 * MT(b, c), 0.5 * MT(b, d)});
+                        if (mx < tauD) continue;
+                        lq++;
+                        const int nA = m.nao[a], nB = m.nao[b], nC = m.nao[c], nD = m.nao[d];
+                        Gam.assign((size_t)nA * nB * nC * nD, 0.0);
+                        for (int A = 0; A < nA; ++A)
+                            for (int B = 0; B < nB; ++B)
+                                for (int C = 0; C < nC; ++C)
+                                    for (int D = 0; D < nD; ++D) {
+                                        const size_t i = m.aoff[a] + A, j = m.aoff[b] + B, k = m.aoff[c] + C,
+                                                     l = m.aoff[d] + D;
+                                        auto p = [&](size_t r, size_t cc) { return P[r + cc * N]; };
+                                        Gam[((size_t)A * nB + B) * nC * nD + (size_t)C * nD + D] =
+                                            p(i, j) * p(k, l) - 0.25 * (p(i, k) * p(j, l) + p(i, l) * p(j, k));
+                                    }
+                        double gq[4][3];
+                        eri_quartet_grad(m, a, b, c, d, Gam.data(), gq);
+                        const double deg = (a != b ? 2.0 : 1.0) * (c != d ? 2.0 : 1.0) * ((a != c || b != d) ? 2.0 : 1.0);
+                        const int s4[4] = {a, b, c, d};
+                        for (int k = 0; k < 4; ++k)
+                            for (int x = 0; x < 3; ++x) gp[3 * shell_atom[s4[k]] + x] += 0.5 * deg * gq[k][x];
+                    }
+#pragma omp critical(oracle_grad)
+        {
+            for (size_t x = 0; x < gt.size(); ++x) gt[x] += gp[x];
+            nq += lq;
+        }
+    }
+    for (size_t x = 0; x < gt.size(); ++x) g[x] += gt[x];
+    if (n_quartets) *n_quartets = nq;
+}
+
+// One-electron gradient terms: -W dS, P dT, P dV (basis centres and the Hellmann-Feynman
+// operator term).  Every ordered shell pair (a, b) is visited, so each matrix element
+// counts once.
+void grad_one(const Mol& m, int nat, const double* Z, const double* xyz, const int* shell_atom, const double* P,
+              const double* W, double* gS, double* gH) {
+    const int N = m.N, nS = m.nS;
+#pragma omp parallel
+    {
+        std::vector<double> lS(3 * (size_t)nat, 0.0), lH(3 * (size_t)nat, 0.0);
+#pragma omp for collapse(2) schedule(dynamic)
+        for (int sa = 0; sa < nS; ++sa)
+            for (int sb = 0; sb < nS; ++sb) {
+                const int nA = m.nao[sa], nB = m.nao[sb];
+                const int lA = m.l[sa], lB = m.l[sb];
+                const int atA = shell_atom[sa], atB = shell_atom[sb];
+                EBuf E[3] = {EBuf(lA + 1, lB + 3), EBuf(lA + 1, lB + 3), EBuf(lA + 1, lB + 3)};
+                const unsigned L = lA + lB + 1;
+                RBuf R(L);
+                std::vector<double> F(L + 1);
+                for (int pa = 0; pa < m.nprim[sa]; ++pa)
+                    for (int pb = 0; pb < m.nprim[sb]; ++pb) {
+                        const double a = m.alpha[m.poff[sa] + pa], b = m.alpha[m.poff[sb] + pb];
+                        const PairData d = pair_data(a, &m.ctr[3 * sa], b, &m.ctr[3 * sb]);
+                        for (int x = 0; x < 3; ++x) hermite(lA + 1, lB + 3, d.p, d.PA[x], d.PB[x], E[x]);
+                        const double cp = m.coeff[m.poff[sa] + pa] * m.coeff[m.poff[sb] + pb];
+                        const double sq = std::pow(M_PI / d.p, 1.5) * d.K * cp;
+                        const double vpref = (2.0 * M_PI / d.p) * d.K * cp;
+                        for (int X = 0; X < nA; ++X) {
+                            const int i = m.aoff[sa] + X;
+                            const int* c1 = &m.cart[3 * i];
+                            const double na = m.norm[m.noff[sa] + pa * nA + X];
+                            for (int Y = 0; Y < nB; ++Y) {
+                                const int j = m.aoff[sb] + Y;
+                                const int* c2 = &m.cart[3 * j];
+                                const double nb = m.norm[m.noff[sb] + pb * nB + Y];
+                                const double w = W[i + (size_t)j * N], pd = P[i + (size_t)j * N];
+                                const double f = na * nb;
+                                // 1-D overlap factor of dimension x with second index jj; der = 0 base,
+                                // 1 d/dA, 2 d/dB.
+                                auto s1 = [&](int x, int jj, int der) {
+                                    return e_der(E[x], c1[x], jj, 0, der, a, b);
+                                };
+                                // kinetic energy of the product with one dimension's factors
+                                // differentiated (dx < 0: none).
+                                auto kin = [&](int dx, int der) {
+                                    double sx[3], tx[3];
+                                    for (int x = 0; x < 3; ++x) {
+                                        const int dd = (x == dx) ? der : 0;
+                                        const int l2 = c2[x];
+                                        sx[x] = s1(x, l2, dd);
+                                        // -1/2 d^2/dx^2 acting on the second function (kinetic_pair)
+                                        double t = b * (2 * l2 + 1) * sx[x] - 2.0 * b * b * s1(x, l2 + 2, dd);
+                                        if (l2 >= 2) t -= 0.5 * l2 * (l2 - 1) * s1(x, l2 - 2, dd);
+                                        tx[x] = t;
+                                    }
+                                    return tx[0] * sx[1] * sx[2] + sx[0] * tx[1] * sx[2] + sx[0] * sx[1] * tx[2];
+                                };
+                                for (int x = 0; x < 3; ++x) {
+                                    for (int der = 1; der <= 2; ++der) {
+                                        double ds = 1.0;
+                                        for (int y = 0; y < 3; ++y) ds *= s1(y, c2[y], y == x ? der : 0);
+                                        const int at = der == 1 ? atA : atB;
+                                        lS[3 * at + x] -= w * f * sq * ds;
+                                        lH[3 * at + x] += pd * f * sq * kin(x, der);
+                                    }
+                                }
+                                // nuclear attraction
+                                for (int C = 0; C < nat; ++C) {
+                                    const double PC[3] = {d.P[0] - xyz[3 * C], d.P[1] - xyz[3 * C + 1],
+                                                          d.P[2] - xyz[3 * C + 2]};
+                                    boys(L, d.p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), F.data());
+                                    hermite_r(d.p, PC, F.data(), R);
+                                    const double zf = -Z[C] * vpref * f * pd;
+                                    double ev[3][3][8];
+                                    int ne[3];
+                                    for (int x = 0; x < 3; ++x) {
+                                        ne[x] = c1[x] + c2[x] + 2;
+                                        for (int der = 0; der < 3; ++der)
+                                            for (int t = 0; t < ne[x]; ++t) ev[der][x][t] = e_der(E[x], c1[x], c2[x], t, der, a, b);
+                                    }
+                                    for (int x = 0; x < 3; ++x) {
+                                        for (int der = 1; der <= 2; ++der) {
+                                            double sum = 0.0;
+                                            for (int t = 0; t < ne[0]; ++t)
+                                                for (int u = 0; u < ne[1]; ++u)
+                                                    for (int v = 0; v < ne[2]; ++v) {
+                                                        const double e0 = (x == 0 ? ev[der][0][t] : ev[0][0][t]);
+                                                        const double e1 = (x == 1 ? ev[der][1][u] : ev[0][1][u]);
+                                                        const double e2 = (x == 2 ? ev[der][2][v] : ev[0][2][v]);
+                                                        sum += e0 * e1 * e2 * R.at(t, u, v, 0);
+                                                    }
+                                            lH[3 * (der == 1 ? atA : atB) + x] += zf * sum;
+                                        }
+                                        // operator derivative: d/dC_x R_tuv(P - C) = -R_{t+1,u,v}
+                                        double sum = 0.0;
+                                        for (int t = 0; t < ne[0] - 1; ++t)
+                                            for (int u = 0; u < ne[1] - 1; ++u)
+                                                for (int v = 0; v < ne[2] - 1; ++v) {
+                                                    const int tt = t + (x == 0), uu = u + (x == 1), vv = v + (x == 2);
+                                                    sum -= ev[0][0][t] * ev[0][1][u] * ev[0][2][v] * R.at(tt, uu, vv, 0);
+                                                }
+                                        lH[3 * C + x] += zf * sum;
+                                    }
+                                }
+                            }
+                        }
+                    }
+            }
+#pragma omp critical(oracle_grad1)
+        for (size_t x = 0; x < lS.size(); ++x) {
+            gS[x] += lS[x];
+            gH[x] += lH[x];
+        }
+    }
+}
+
+void grad_nuc(int nat, const double* Z, const double* xyz, double* g) {
+    for (int A = 0; A < nat; ++A)
+        for (int B = 0; B < nat; ++B) {
+            if (A == B) continue;
+            const double d[3] = {xyz[3 * A] - xyz[3 * B], xyz[3 * A + 1] - xyz[3 * B + 1], xyz[3 * A + 2] - xyz[3 * B + 2]};
+            const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            for (int x = 0; x < 3; ++x) g[3 * A + x] -= Z[A] * Z[B] * d[x] / (r * r * r);
+        }
+}
+
 }  // namespace
 
 extern "C" {
@@ -893,6 +1217,45 @@ static int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st, 0, 1 << 30);
+}
+
+int hfx_rhf_gradient(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts, hfx_grad_stats* st) {
+    if (!e || !a || !grad || !a->P || !a->W || a->n_atoms <= 0 || !a->Z || !a->xyz)
+        return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: null argument");
+    if (e->mols.size() != 1) return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: single-molecule engines only");
+    if (a->flags & HFX_JK_DEVICE_PTRS) return fail(HFX_ERR_INVALID, "oracle: host pointers only");
+    Mol& m = e->mols[0];
+    const int nat = a->n_atoms;
+    if (!all_finite(a->P, e->elems) || !all_finite(a->W, e->elems))
+        return fail(HFX_ERR_NONFINITE, "hfx_rhf_gradient: non-finite density");
+    std::vector<int> sat(m.nS);
+    for (int s = 0; s < m.nS; ++s) {
+        if (a->shell_atom) {
+            sat[s] = a->shell_atom[s];
+        } else {
+            sat[s] = -1;
+            for (int A = 0; A < nat && sat[s] < 0; ++A)
+                if (m.ctr[3 * s] == a->xyz[3 * A] && m.ctr[3 * s + 1] == a->xyz[3 * A + 1] &&
+                    m.ctr[3 * s + 2] == a->xyz[3 * A + 2])
+                    sat[s] = A;
+        }
+        if (sat[s] < 0 || sat[s] >= nat) return fail(HFX_ERR_INVALID, "hfx_rhf_gradient: shell not on an atom");
+    }
+    const size_t n3 = 3 * (size_t)nat;
+    std::vector<double> g[4];
+    for (auto& v : g) v.assign(n3, 0.0);
+    grad_nuc(nat, a->Z, a->xyz, g[0].data());
+    grad_one(m, nat, a->Z, a->xyz, sat.data(), a->P, a->W, g[1].data(), g[2].data());
+    int64_t nq = 0;
+    grad_eri(m, a->P, sat.data(), a->schwarz_thresh, a->density_thresh, g[3].data(), &nq);
+    for (size_t x = 0; x < n3; ++x) grad[x] = g[0][x] + g[1][x] + g[2][x] + g[3][x];
+    if (parts)
+        for (int k = 0; k < 4; ++k) std::copy(g[k].begin(), g[k].end(), parts + k * n3);
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_quartets = nq;
+    }
+    return HFX_OK;
 }
 
 // --- oracle-only entry points (tests) -----------------------------------------------------

@@ -1,0 +1,97 @@
+"""RHF nuclear gradient of the HIP engine: against the oracle's analytic gradient on the same
+densities, against finite differences of SCF energies, in mixed precision, and through a
+torch.autograd.Function."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gradient_tools as gt
+import molecules
+from hfx import Basis, geometry, lib
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _water8():
+    Z, xyz = geometry.water_cluster_grid(shape=(2, 2, 2), spacing=5.6, jitter=0.4, seed=1002)
+    return Basis("6-31g**", Z, xyz)
+
+
+@pytest.mark.parametrize("mol", ["h2o_sto3g", "h2o_631gss", "dimer_631gss", "ethane"])
+def test_gradient_matches_oracle(engine_lib, oracle, mol):
+    b = {"h2o_sto3g": lambda: molecules.h2o("sto-3g"), "h2o_631gss": lambda: molecules.h2o("6-31g**"),
+         "dimer_631gss": lambda: molecules.water_dimer("6-31g**"), "ethane": molecules.ethane}[mol]()
+    print("gradient vs oracle", mol, b.n_aos, flush=True)
+    r = gt.scf(oracle, b, full_builds=0, energy_tol=1e-9, density_tol=1e-7)
+    P, W = gt.densities(r, b)
+    kw = dict(schwarz_thresh=1e-12, density_thresh=1e-12)
+    gg, pg, sg = lib.rhf_gradient(lib.Engine(engine_lib, b), b, P, W, **kw)
+    go, po, so = lib.rhf_gradient(lib.Engine(oracle, b), b, P, W, **kw)
+    assert sg["n_quartets"] == so["n_quartets"]
+    for k in range(4):
+        assert np.abs(pg[k] - po[k]).max() <= 1e-9, k
+    assert np.abs(gg - go).max() <= 1e-9
+
+
+def test_water8_gradient_against_finite_differences(engine_lib):
+    """All 72 coordinates of (H2O)8 / 6-31G**: central differences (h = 1e-4 Bohr) of the
+    engine's SCF energies, the 144 displaced molecules run as one batched SCF."""
+    b = _water8()
+    print("water8 gradient vs FD", b.n_aos, flush=True)
+    kw = dict(full_builds=0, energy_tol=1e-10, density_tol=1e-8, schwarz_thresh=1e-12, density_thresh=1e-12)
+    r, g, parts, st = gt.analytic(engine_lib, b, **kw)
+    fd = gt.finite_differences_batched(engine_lib, b, **kw)
+    err = np.abs(fd - g).max()
+    print("max |analytic - FD| = %.2e Eh/Bohr, max |g| = %.2e" % (err, np.abs(g).max()), flush=True)
+    assert err <= 1e-6
+    assert np.abs(g.sum(0)).max() <= 1e-9
+
+
+def test_water8_gradient_against_oracle_fixture(engine_lib):
+    """The oracle's FP64 analytic gradient and its finite differences, stored in
+    tests/golden/water8_grad.json."""
+    path = os.path.join(HERE, "golden", "water8_grad.json")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    try:
+        ref = json.load(open(path))
+    except ValueError:
+        pytest.skip("fixture incomplete")
+    b = _water8()
+    kw = dict(**ref["scf"])
+    r, g, parts, st = gt.analytic(engine_lib, b, **kw)
+    assert abs(r["e_total"] - ref["e_total"]) <= 1e-9
+    assert np.abs(g - np.asarray(ref["grad"])).max() <= 1e-7
+    for A, x, fd in ref["fd"]:
+        assert abs(g[A, x] - fd) <= 1e-6, (A, x)
+
+
+def test_gradient_mixed_precision(engine_lib):
+    b = _water8()
+    print("mixed gradient", flush=True)
+    r = gt.scf(engine_lib, b, full_builds=0, energy_tol=1e-9, density_tol=1e-7)
+    P, W = gt.densities(r, b)
+    e = lib.Engine(engine_lib, b)
+    g64, _, s64 = lib.rhf_gradient(e, b, P, W)
+    gmx, _, smx = lib.rhf_gradient(e, b, P, W, precision=lib.HFX_MIXED, fp32_cut=1e-5)
+    assert 0 < smx["n_fp32"] < smx["n_quartets"] == s64["n_quartets"]
+    assert np.abs(gmx - g64).max() <= 1e-6
+
+
+def test_autograd_function(engine_lib):
+    """The SCF energy as a torch function of the coordinates: backward gives the analytic
+    gradient, which matches finite differences."""
+    torch = pytest.importorskip("torch")
+    b = molecules.water_dimer("6-31g**")
+    kw = dict(full_builds=0, energy_tol=1e-10, density_tol=1e-8, schwarz_thresh=1e-12, density_thresh=1e-12)
+    F = gt.scf_energy_function(engine_lib, b.name, b.Z, **kw)
+    x = torch.tensor(b.xyz, dtype=torch.float64, requires_grad=True)
+    E = F.apply(x)
+    E.backward()
+    fd = gt.finite_differences(engine_lib, b, coords=[(0, 0), (1, 1), (3, 2), (5, 0)], **kw)
+    for (A, c), v in fd.items():
+        assert abs(x.grad[A, c].item() - v) <= 1e-6

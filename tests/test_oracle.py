@@ -379,3 +379,45 @@ def test_model_flop_table(oracle):
     assert f(1, 1, 1, 1) == 1661
     assert f(2, 1, 2, 1) == 9527
     assert f(2, 2, 2, 2) == 51836
+
+
+# ---------------------------------------------------------------------------------------
+# RHF nuclear gradient of the oracle: pinned by central finite differences of its own SCF
+# energies and by the invariances of the exact gradient.
+# ---------------------------------------------------------------------------------------
+def _distorted_water(name):
+    from hfx import Basis
+    b = molecules.h2o(name)
+    xyz = b.xyz.copy()
+    xyz[0] += (0.12, -0.07, 0.05)
+    xyz[2] += (-0.04, 0.09, -0.03)
+    return Basis(name, b.Z, xyz)
+
+
+def test_gradient_matches_finite_differences(oracle):
+    import gradient_tools as gt
+    b = _distorted_water("6-31g**")
+    r, g, parts, _ = gt.analytic(oracle, b)
+    fd = gt.finite_differences(oracle, b, coords=[(0, 0), (0, 2), (1, 1), (2, 0), (2, 2)])
+    err = max(abs(fd[k] - g[k]) for k in fd)
+    assert err <= 1e-6, err
+    assert np.abs(g).max() > 1e-2  # a real test: the distorted geometry has sizable forces
+
+
+def test_gradient_invariances(oracle):
+    import gradient_tools as gt
+    b = molecules.water_dimer("6-31g**")
+    r, g, parts, _ = gt.analytic(oracle, b)
+    # translation: every part sums to zero over atoms on its own
+    for k in range(4):
+        assert np.abs(parts[k].sum(0)).max() <= 1e-9, k
+    # rotation: the total torque of the exact gradient vanishes
+    torque = np.cross(b.xyz, g).sum(0)
+    assert np.abs(torque).max() <= 1e-7
+    # nuclear repulsion part against the closed form
+    Z = np.asarray(b.Z, float)
+    d = b.xyz[:, None, :] - b.xyz[None, :, :]
+    r3 = np.linalg.norm(d, axis=-1) ** 3
+    np.fill_diagonal(r3, np.inf)
+    gn = -(Z[:, None, None] * Z[None, :, None] * d / r3[:, :, None]).sum(1)
+    assert np.abs(parts[0] - gn).max() <= 1e-12
