@@ -94,7 +94,7 @@ def cpu_baseline(basis, Da, Q, total_gflop, budget_s):
     mid = nS // 2
     t, g = run(mid, mid + 1)  # calibrate on one bra shell
     width = 1
-    while t < budget_s / 8 and mid + width < nS:
+    while t < budget_s / 3 and mid + width < nS:
         width *= 2
         t, g = run(mid, min(nS, mid + width))
     rate = g / t if t > 0 else 0.0  # model GFLOP/s on this slice
@@ -145,16 +145,32 @@ def gradient_run(L):
                 max_dgrad_mixed=float(np.abs(gmx - g64).max()), max_abs_grad=float(np.abs(g64).max()))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eri_c20h42_mixed_traffic.json")
+
+
+def measured_traffic(group):
+    """HBM bytes per launch of a class kernel group from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of the C20H42 mixed build (scaled by the copy-kernel calibration recorded there)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)["per_class_launch_group"].get(group)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if t is None else t["read_bytes"] + t["write_bytes"]
+
+
 def roofline_of(stmx, cm64, cm32):
     """Roofline of the dominant class kernel group (one class, one precision) of a build."""
     groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
     ms, prec, k = max(groups)
     gfl = (stmx["class_gflop64"] if prec == "f64" else stmx["class_gflop32"])[k]
     achieved = gfl / ms if ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
-    return dict(bound="mfma", pipe="VALU %s (MI355X peak listed for %s)" % (prec, prec), kernel="k_eri %s %s" % (
-        CLASS_NAMES[k], prec), achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
-        frac=achieved / PEAK_TFLOPS[prec], traffic=None, launch_ms=ms, launch_model_gflop=gfl,
-        build_achieved_tflops=stmx["model_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
+    group = "%s %s" % (CLASS_NAMES[k], prec)
+    return dict(bound="mfma", pipe="VALU %s (MI355X peak listed for %s)" % (prec, prec), kernel="k_eri " + group,
+                achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s", frac=achieved / PEAK_TFLOPS[prec],
+                traffic=measured_traffic(group), traffic_source="rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch, "
+                "profiles/k_eri_c20h42_mixed_traffic.md", launch_ms=ms, launch_model_gflop=gfl,
+                build_achieved_tflops=stmx["model_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
 
 
 def profile_builds(args):
@@ -180,7 +196,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--carbons", type=int, default=20)
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--save-density", default=None)
     ap.add_argument("--density", default=None)

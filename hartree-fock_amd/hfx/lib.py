@@ -385,10 +385,11 @@ _LIB = None
 
 
 def load(path=None):
-    """The HIP engine library.  Raises HfxError when it has not been built."""
+    """The HIP engine library (lib/libhfx_mi355x.so, or the file HFX_LIBRARY names).
+    Raises HfxError when it has not been built."""
     global _LIB
     if path is not None:
         return Library(path)
     if _LIB is None:
-        _LIB = Library(LIB_PATH)
+        _LIB = Library(os.environ.get("HFX_LIBRARY") or LIB_PATH)
     return _LIB
