@@ -138,6 +138,10 @@ __global__ void k_fill_u64(unsigned long long* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
+__global__ void k_fill_u32(unsigned int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 struct DmaxArgs {
     const int* pair_a;
     const int* pair_b;
@@ -376,17 +380,37 @@ __global__ void k_ranges(const unsigned long long* offs, const long long* slots,
 
 // out = sum over the accumulator copies of A + A^T, within every molecule block
 // (blockIdx.y = molecule).
+// Sum of the per-XCD packed-triangle copies, expanded to the full symmetric block; the
+// diagonal holds one orientation only and is doubled.
 __global__ void k_symm(const double* A, long long stride, int copies, double* out, const long long* mbase,
-                       const int* mn) {
+                       const long long* tbase, const int* mn) {
     const int m = blockIdx.y;
-    const long long base = mbase[m];
+    const long long base = mbase[m], tb = tbase[m];
     const long long N = mn[m];
     const long long n = N * N;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
         const long long r = e % N, c = e / N;
+        const long long t = tb + tri_index(r, c);
         double v = 0.0;
-        for (int k = 0; k < copies; ++k) v += A[k * stride + base + r + c * N] + A[k * stride + base + c + r * N];
-        out[base + e] = v;
+        for (int k = 0; k < copies; ++k) v += A[k * stride + t];
+        out[base + e] = (r == c) ? 2.0 * v : v;
+    }
+}
+
+// Packed lower triangle of each molecule's block of a symmetric matrix.
+__global__ void k_pack_tri(const double* __restrict__ D, double* __restrict__ T, const long long* mbase,
+                           const long long* tbase, const int* mn) {
+    const int m = blockIdx.y;
+    const long long base = mbase[m], tb = tbase[m];
+    const long long N = mn[m];
+    const long long n = N * (N + 1) / 2;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        // e = i (i + 1) / 2 + j with j <= i
+        long long i = (long long)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > e) --i;
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        const long long j = e - i * (i + 1) / 2;
+        T[tb + e] = D[base + i + j * N];
     }
 }
 
@@ -579,13 +603,13 @@ struct hfx_engine {
     // molecules
     int n_mol = 0;
     std::vector<int> mol_N, mol_nS, mol_shell0, mol_atom0;
-    std::vector<long long> mol_mbase, mol_qbase;
-    long long n_elems = 0, n_qelems = 0;
+    std::vector<long long> mol_mbase, mol_qbase, mol_tbase;
+    long long n_elems = 0, n_qelems = 0, n_telems = 0;
     int nS = 0, N_tot = 0;
 
     // shells (global index), host
     std::vector<int> sh_l, sh_np, sh_po, sh_no, sh_ao, sh_nao, sh_mol, sh_local, sh_ns, sh_mn;
-    std::vector<long long> sh_mbase, sh_qbase;
+    std::vector<long long> sh_mbase, sh_qbase, sh_tbase;
     std::vector<double> sh_ctr, sh_cf, alpha, coeff, norm;
 
     // pairs
@@ -624,7 +648,7 @@ struct hfx_engine {
     long long gpart_cap = 0;
     int *d_sh_ao = nullptr, *d_sh_nao = nullptr, *d_sh_mn = nullptr, *d_sh_local = nullptr, *d_sh_ns = nullptr;
     int *d_sh_po = nullptr, *d_sh_np = nullptr, *d_sh_no = nullptr, *d_sh_mol = nullptr;
-    long long *d_sh_mbase = nullptr, *d_sh_qbase = nullptr;
+    long long *d_sh_mbase = nullptr, *d_sh_qbase = nullptr, *d_sh_tbase = nullptr;
     double *d_sh_cf = nullptr, *d_ctr = nullptr, *d_alpha = nullptr, *d_coeff = nullptr, *d_norm = nullptr;
     int* d_cls_ids[N_PAIR_CLASSES] = {};
     double* d_cls_q[N_PAIR_CLASSES] = {};
@@ -632,13 +656,15 @@ struct hfx_engine {
     double* d_boys64 = nullptr;
     float* d_boys32 = nullptr;
     long long* d_mbase = nullptr;
+    long long* d_tbase = nullptr;
     int* d_mn = nullptr;
     long long* d_reg = nullptr;
 
     // per-build buffers
     double *d_Dt = nullptr, *d_Da = nullptr, *d_Db = nullptr;
+    double *d_Tt = nullptr, *d_Ta = nullptr, *d_Tb = nullptr;  // packed triangles of d_Dt, d_Da, d_Db
     double *d_Mt = nullptr, *d_Ma = nullptr, *d_Mb = nullptr;
-    double *d_AJ = nullptr, *d_AKa = nullptr, *d_AKb = nullptr;  // ACC_COPIES copies each
+    double *d_AJ = nullptr, *d_AKa = nullptr, *d_AKb = nullptr;  // ACC_COPIES packed-triangle copies each
     static constexpr int ACC_COPIES = 8;  // one per XCD of an MI355X
     double *d_J = nullptr, *d_Ka = nullptr, *d_Kb = nullptr;
     uint2* d_list = nullptr;
@@ -656,6 +682,10 @@ struct hfx_engine {
     hipGraphExec_t graph_exec = nullptr;
     BuildParams graph_key{-1.0, -1.0, -1.0, false, false, false};
     long long* d_ranges = nullptr;          // [4*N_COMBOS]
+    unsigned int* d_chunk_ctr = nullptr;     // [2*N_COMBOS*CHUNK_SLOTS]: chunk counters per class kernel
+    static constexpr int CHUNK_SLOTS = 64;   // chunk kernels per class, at most
+    static constexpr int CHUNK_LEN = 2048;   // list entries a block claims at a time
+    int max_N = 0;
 
     ~hfx_engine() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -779,6 +809,11 @@ EriArgs eri_args(hfx_engine* e) {
     a.pp32 = e->d_pp32;
     a.sh_ao = e->d_sh_ao;
     a.sh_mbase = e->d_sh_mbase;
+    a.sh_tbase = e->d_sh_tbase;
+    a.sh_local = e->d_sh_local;
+    a.sh_nao = e->d_sh_nao;
+    a.sh_ns = e->d_sh_ns;
+    a.tile_shells = e->ns_max;
     a.sh_mn = e->d_sh_mn;
     a.sh_cf = e->d_sh_cf;
     a.boys64 = e->d_boys64;
@@ -889,6 +924,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
         e->mol_nS.push_back(b->n_shells);
         e->mol_mbase.push_back(e->n_elems);
         e->mol_qbase.push_back(e->n_qelems);
+        e->mol_tbase.push_back(e->n_telems);
         int ao = 0;
         for (int s = 0; s < b->n_shells; ++s) {
             const int l = b->l[s];
@@ -913,6 +949,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
             e->sh_mn.push_back(b->n_aos);
             e->sh_mbase.push_back(e->n_elems);
             e->sh_qbase.push_back(e->n_qelems);
+            e->sh_tbase.push_back(e->n_telems);
             for (int x = 0; x < 3; ++x) e->sh_ctr.push_back(b->centers[3 * s + x]);
             // normalization ratio of each component (independent of the primitive)
             const double* nrm = &b->norm[b->norm_offset[s]];
@@ -931,6 +968,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
         e->N_tot += b->n_aos;
         e->n_elems += (long long)b->n_aos * b->n_aos;
         e->n_qelems += (long long)b->n_shells * b->n_shells;
+        e->n_telems += (long long)b->n_aos * (b->n_aos + 1) / 2;
     }
 
     // ---- shell pairs and primitive-pair data ----
@@ -1050,6 +1088,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
         (rc = upload(e.get(), e->d_sh_po, e->sh_po)) || (rc = upload(e.get(), e->d_sh_np, e->sh_np)) ||
         (rc = upload(e.get(), e->d_sh_no, e->sh_no)) || (rc = upload(e.get(), e->d_sh_mol, e->sh_mol)) ||
         (rc = upload(e.get(), e->d_sh_mbase, e->sh_mbase)) || (rc = upload(e.get(), e->d_sh_qbase, e->sh_qbase)) ||
+        (rc = upload(e.get(), e->d_sh_tbase, e->sh_tbase)) || (rc = upload(e.get(), e->d_tbase, e->mol_tbase)) ||
         (rc = upload(e.get(), e->d_sh_cf, e->sh_cf)) || (rc = upload(e.get(), e->d_ctr, e->sh_ctr)) ||
         (rc = upload(e.get(), e->d_alpha, e->alpha)) || (rc = upload(e.get(), e->d_coeff, e->coeff)) ||
         (rc = upload(e.get(), e->d_norm, e->norm)) || (rc = upload(e.get(), e->d_boys64, b64)) ||
@@ -1065,9 +1104,13 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     e->d_Mt = e->mem.alloc<double>(nq);
     e->d_Ma = e->mem.alloc<double>(nq);
     e->d_Mb = e->mem.alloc<double>(nq);
-    e->d_AJ = e->mem.alloc<double>(ne * hfx_engine::ACC_COPIES);
-    e->d_AKa = e->mem.alloc<double>(ne * hfx_engine::ACC_COPIES);
-    e->d_AKb = e->mem.alloc<double>(ne * hfx_engine::ACC_COPIES);
+    const long long nt = e->n_telems;
+    e->d_Tt = e->mem.alloc<double>(nt);
+    e->d_Ta = e->mem.alloc<double>(nt);
+    e->d_Tb = e->mem.alloc<double>(nt);
+    e->d_AJ = e->mem.alloc<double>(nt * hfx_engine::ACC_COPIES);
+    e->d_AKa = e->mem.alloc<double>(nt * hfx_engine::ACC_COPIES);
+    e->d_AKb = e->mem.alloc<double>(nt * hfx_engine::ACC_COPIES);
     e->d_J = e->mem.alloc<double>(ne);
     e->d_Ka = e->mem.alloc<double>(ne);
     e->d_Kb = e->mem.alloc<double>(ne);
@@ -1089,9 +1132,11 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     e->d_scan_tmp = e->mem.alloc<char>(e->scan_tmp_bytes);
     e->d_stats = e->mem.alloc<unsigned long long>(4 * N_COMBOS);
     e->d_ranges = e->mem.alloc<long long>(4 * N_COMBOS);
-    if (!e->d_Dt || !e->d_Da || !e->d_Db || !e->d_Mt || !e->d_Ma || !e->d_Mb || !e->d_AJ || !e->d_AKa || !e->d_AKb ||
+    e->d_chunk_ctr = e->mem.alloc<unsigned int>(2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
+    for (int m = 0; m < n_mol; ++m) e->max_N = std::max(e->max_N, e->mol_N[m]);
+    if (!e->d_Tt || !e->d_Ta || !e->d_Tb || !e->d_Dt || !e->d_Da || !e->d_Db || !e->d_Mt || !e->d_Ma || !e->d_Mb || !e->d_AJ || !e->d_AKa || !e->d_AKb ||
         !e->d_J || !e->d_Ka || !e->d_Kb || !e->d_list || !e->d_counts || !e->d_offs ||
-        !e->d_scan_tmp || !e->d_stats || !e->d_ranges)
+        !e->d_scan_tmp || !e->d_stats || !e->d_ranges || !e->d_chunk_ctr)
         return fail(HFX_ERR_NOMEM, "hipMalloc failed for the build buffers");
 
     // ---- Schwarz table, then the class order it implies ----
@@ -1195,33 +1240,55 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         const int rc = enqueue_screen(e, P, s);
         if (rc) return rc;
     }
+    const long long nt = e->n_telems;
     {
         const int g = e->n_cu * 4;
-        const long long nc = ne * hfx_engine::ACC_COPIES;
+        const long long nc = nt * hfx_engine::ACC_COPIES;
         hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, nc, 0.0);
         hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, nc, 0.0);
         if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, nc, 0.0);
+        const dim3 gp(std::max(1, std::min(e->n_cu * 4, (int)((nt / e->n_mol + 255) / 256))), e->n_mol);
+        hipLaunchKernelGGL(k_pack_tri, gp, dim3(256), 0, s, e->d_Dt, e->d_Tt, e->d_mbase, e->d_tbase, e->d_mn);
+        hipLaunchKernelGGL(k_pack_tri, gp, dim3(256), 0, s, e->d_Da, e->d_Ta, e->d_mbase, e->d_tbase, e->d_mn);
+        if (uhf) hipLaunchKernelGGL(k_pack_tri, gp, dim3(256), 0, s, e->d_Db, e->d_Tb, e->d_mbase, e->d_tbase, e->d_mn);
     }
     EriArgs ea = eri_args(e);
     ea.list = e->d_list;
-    ea.Dt = e->d_Dt;
-    ea.Da = e->d_Da;
-    ea.Db = uhf ? e->d_Db : nullptr;
+    ea.Dt = e->d_Tt;
+    ea.Da = e->d_Ta;
+    ea.Db = uhf ? e->d_Tb : nullptr;
     ea.AJ = e->d_AJ;
     ea.AKa = e->d_AKa;
     ea.AKb = e->d_AKb;
-    ea.acc_stride = ne;
+    ea.acc_stride = nt;
     ea.n_copies = hfx_engine::ACC_COPIES;
     ea.tauS = P.tauS;
     ea.out = nullptr;
     ea.out_off = nullptr;
     const int grid = eri_grid(e);
+    hipLaunchKernelGGL(k_fill_u32, dim3(4), dim3(256), 0, s, e->d_chunk_ctr, 2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
+    ea.chunk_len = hfx_engine::CHUNK_LEN;
+    // LDS exchange rows of a class kernel: NA + NB rows of max_N doubles (twice for UHF), next
+    // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without,
+    // and so does (ss|ss), whose four exchange terms per quartet cost less than the rows.
+    auto tile_cols = [&](int k, int fp32) {
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
+        if (L == 0) return 0;
+        const int nt = fp32 ? 5 : 8, nord = L <= 1 ? L + nt : nt;
+        const size_t boys = (size_t)md::BOYS_POINTS * nord * (fp32 ? 4 : 8);
+        const size_t rows = (size_t)(uhf ? 2 : 1) * (ncart_h(la) + ncart_h(lb)) * e->max_N * sizeof(double);
+        const size_t mask = (size_t)((e->ns_max + 31) / 32) * sizeof(unsigned int);
+        return boys + rows + mask + 1024 <= 64 * 1024 ? e->max_N : 0;
+    };
     if (timing) HIPCHK(hipEventRecord(e->cev[n_ev++], s));
     for (int k = 0; k < N_COMBOS; ++k) {
         ev_slot[2 * k] = ev_slot[2 * k + 1] = -1;
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
         ea.range = e->d_ranges + 2 * k;
+        ea.chunk_ctr = e->d_chunk_ctr + (2 * k) * hfx_engine::CHUNK_SLOTS;
+        ea.tile_n = tile_cols(k, 0);
         launch_eri(k, 0, ea, grid, s);
         if (timing) {
             HIPCHK(hipEventRecord(e->cev[n_ev], s));
@@ -1229,6 +1296,8 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         }
         if (mixed) {
             ea.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
+            ea.chunk_ctr = e->d_chunk_ctr + (2 * k + 1) * hfx_engine::CHUNK_SLOTS;
+            ea.tile_n = tile_cols(k, 1);
             launch_eri(k, 1, ea, grid, s);
             if (timing) {
                 HIPCHK(hipEventRecord(e->cev[n_ev], s));
@@ -1239,9 +1308,10 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     {
         const dim3 g(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256))), e->n_mol);
         const int cp = hfx_engine::ACC_COPIES;
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, ne, cp, e->d_J, e->d_mbase, e->d_mn);
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, ne, cp, e->d_Ka, e->d_mbase, e->d_mn);
-        if (uhf) hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, ne, cp, e->d_Kb, e->d_mbase, e->d_mn);
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, nt, cp, e->d_J, e->d_mbase, e->d_tbase, e->d_mn);
+        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, nt, cp, e->d_Ka, e->d_mbase, e->d_tbase, e->d_mn);
+        if (uhf)
+            hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, nt, cp, e->d_Kb, e->d_mbase, e->d_tbase, e->d_mn);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipGetLastError());

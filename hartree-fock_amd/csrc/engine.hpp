@@ -12,7 +12,9 @@
 //   work lists   uint2 {bra pair, ket pair} per surviving shell quartet, one region per
 //                quartet class: FP64 entries fill the region from its start, FP32 entries
 //                from its end
-//   matrices     N x N column-major blocks, one per molecule, concatenated
+//   matrices     N x N column-major blocks, one per molecule, concatenated; the kernels'
+//                density reads and J/K accumulators use packed lower triangles of the same
+//                blocks (N(N+1)/2 each, concatenated), which halves their cache footprint
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,6 +45,11 @@ __host__ __device__ constexpr int combo_pc2(int k) { return k - combo_pc1(k) * (
 __host__ __device__ constexpr int pc_la(int pc) { return combo_pc1(pc); }
 __host__ __device__ constexpr int pc_lb(int pc) { return combo_pc2(pc); }
 
+// Packed lower triangle of a symmetric block: element (i, j) of either orientation.
+__host__ __device__ __forceinline__ long long tri_index(long long i, long long j) {
+    return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
 constexpr int N_PAIR_CLASSES = 6;
 constexpr int N_COMBOS = 21;
 
@@ -57,16 +64,17 @@ struct EriArgs {
     const PrimPair32* pp32;
     const int* sh_ao;
     const long long* sh_mbase;
+    const long long* sh_tbase;  // offset of the shell's molecule in the packed triangles
     const int* sh_mn;
     const double* sh_cf;
     const uint2* list;
     const long long* range;  // [lo, hi) of this class's entries in list
-    const double* Dt;
+    const double* Dt;  // packed triangles of the density increments
     const double* Da;
     const double* Db;  // NULL: no K_beta
-    double* AJ;    // accumulators: n_copies copies, acc_stride elements apart; each
-    double* AKa;   // XCD adds into its own copy (selected by the XCC_ID hardware register)
-    double* AKb;
+    double* AJ;    // packed-triangle accumulators: n_copies copies, acc_stride elements apart;
+    double* AKa;   // each XCD adds into its own copy (selected by the XCC_ID hardware
+    double* AKb;   // register); element (i,j) holds the sum of both orientations
     long long acc_stride;
     int n_copies;
     double tauS;
@@ -74,6 +82,17 @@ struct EriArgs {
     const long long* out_off;  // store mode: offset of each entry's block
     const double* boys64;
     const float* boys32;
+    // Chunked schedule (NULL: grid-stride over the list): blocks claim chunk_len consecutive
+    // entries at a time from chunk_ctr[chunk index of the launch], so a block stays on one bra
+    // pair for many quartets and collects that pair's exchange rows in LDS (tile_n columns
+    // per row; 0: no LDS rows).
+    unsigned int* chunk_ctr;
+    int chunk_len;     // largest chunk; smaller lists use shorter chunks
+    int tile_n;
+    int tile_shells;   // bits of the touched-ket-shell mask kept with the rows
+    const int* sh_local;  // index of a shell inside its molecule
+    const int* sh_nao;
+    const int* sh_ns;     // shells of the shell's molecule
 };
 
 // Arguments of the RHF-gradient kernels (k_grad.hip).

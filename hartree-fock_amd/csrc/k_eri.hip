@@ -12,13 +12,21 @@
 // the contracted integrals.  This is the primitive integral of the reference's
 // electronRepulsion (src/IntegralEngine.cpp:897-934) in factored order.
 //
+// Schedule: blocks claim contiguous chunks of the class's list (bra-major), so a block keeps
+// one bra pair (a,b) for many consecutive quartets.  The exchange rows of that pair,
+// K[a,:] and K[b,:], are collected in LDS with LDS atomics, with a mask of the ket shells
+// touched, and the touched columns are added to the accumulators once when the block moves
+// to another bra pair; quartets of other bra pairs in the same iteration, and the Coulomb
+// terms of the ket pairs, go to the accumulators directly.
+//
 // Digestion: the full block of the quartet is used with the permutational degeneracy
 // s = (a==b ? 1/2 : 1)(c==d ? 1/2 : 1)(ab==cd ? 1/2 : 1); the integrals below tau_S are
 // dropped as the reference's AO test (src/SCF.cpp:670-671).  Contributions go to
 //   AJ[i,j] += 2 s I D[k,l],  AJ[k,l] += 2 s I D[i,j],
 //   AK[i,k] += s I D[j,l], AK[i,l] += s I D[j,k], AK[j,k] += s I D[i,l], AK[j,l] += s I D[i,k]
-// accumulated per thread in registers and flushed with FP64 atomics; J = AJ + AJ^T and
-// K = AK + AK^T afterwards.  This equals the reference's canonical-AO digestion
+// accumulated per thread in registers and flushed with FP64 atomics into packed lower
+// triangles (both orientations of (i,j) land in one element, i.e. J = AJ + AJ^T is formed
+// on the fly; the diagonal is doubled when the triangles are expanded).  This equals the reference's canonical-AO digestion
 // (src/SCF.cpp:633-711) summed over the 8 permutations.
 #include "engine.hpp"
 #include "md.hpp"
@@ -65,6 +73,10 @@ __device__ __forceinline__ const float* boys_of<float>(const EriArgs& a) { retur
 __device__ __forceinline__ void atomic_add(double* p, double v) {
     if (v != 0.0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Accumulation into the block's LDS exchange rows.
+__device__ __forceinline__ void lds_add(double* p, double v) {
+    if (v != 0.0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Index of the XCD running this workgroup (hardware register XCC_ID, bits 3:0).
 __device__ __forceinline__ int xcd_id() { return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u); }
@@ -81,8 +93,11 @@ __host__ __device__ constexpr bool chunk_has_b(int c0, int ib) {
     return false;
 }
 
+// tile: the block's LDS exchange rows (NA + NB rows of a.tile_n, then the same for K_beta),
+// holding bra pair tile_pid; NULL when the block keeps no rows.
 template <class R, int C0>
-__device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab) {
+__device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab, double* tile,
+                                        int tile_pid) {
     const uint2 e = a.list[qi];
     const int pid = (int)e.x, qid = (int)e.y;
     const int* __restrict__ pps = sizeof(R) == 8 ? a.pair_ps : a.pair_ps32;
@@ -199,8 +214,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
         });
         return;
     } else {
-        const long long base = a.sh_mbase[sa];
-        const long long N = a.sh_mn[sa];
+        const long long base = a.sh_tbase[sa];
         const int oA = a.sh_ao[sa], oB = a.sh_ao[sb], oC = a.sh_ao[sc], oD = a.sh_ao[sd];
         double s = 1.0;
         if (sa == sb) s *= 0.5;
@@ -227,7 +241,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             sfor<NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
                 const long long k = oC + cd / ND, l = oD + cd % ND;
-                const double dkl = Dt[k + l * N];
+                const double dkl = Dt[tri_index(k, l)];
                 sfor<CB>([&](auto c_) {
                     constexpr int c = decltype(c_)::value;
                     constexpr int ab = C0 + c;
@@ -235,7 +249,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                         const long long i = oA + ab / NB, j = oB + ab % NB;
                         const double v = val(c_, cd_);
                         Jab[c] += v * dkl;
-                        Jcd[cd] += v * Dt[i + j * N];
+                        Jcd[cd] += v * Dt[tri_index(i, j)];
                     }
                 });
             });
@@ -264,16 +278,19 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                         const double o = __shfl_down(x, off);
                         if (same[decltype(k_)::value]) x += o;
                     });
-                    if (head && valid) atomic_add(&AJ[(oA + ab / NB) + (oB + ab % NB) * N], 2.0 * x);
+                    if (head && valid) atomic_add(&AJ[tri_index(oA + ab / NB, oB + ab % NB)], 2.0 * x);
                 }
             });
             sfor<NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
-                atomic_add(&AJ[(oC + cd / ND) + (oD + cd % ND) * N], 2.0 * Jcd[cd]);
+                atomic_add(&AJ[tri_index(oC + cd / ND, oD + cd % ND)], 2.0 * Jcd[cd]);
             });
         }
-        // Exchange with one spin density Dx into the accumulator AKx.
-        auto exchange = [&](const double* __restrict__ Dx, double* AKx) {
+        // Exchange with one spin density Dx into the accumulator AKx (rows of a and b into the
+        // LDS rows T instead when the block holds this bra pair).
+        const bool in_tile = tile != nullptr && pid == tile_pid;
+        const int tn = a.tile_n;
+        auto exchange = [&](const double* __restrict__ Dx, double* AKx, double* T) {
             double Kac[NA][NC], Kad[NA][ND], Kbc[NB][NC], Kbd[NB][ND];
             sfor<NA>([&](auto x) {
                 sfor<NC>([&](auto y) { Kac[x][y] = 0.0; });
@@ -294,23 +311,52 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                         constexpr int ia = ab / NB, ib = ab % NB;
                         const long long i = oA + ia, j = oB + ib;
                         const double v = val(c_, cd_);
-                        Kac[ia][jc] += v * Dx[j + l * N];
-                        Kad[ia][jd] += v * Dx[j + k * N];
-                        Kbc[ib][jc] += v * Dx[i + l * N];
-                        Kbd[ib][jd] += v * Dx[i + k * N];
+                        Kac[ia][jc] += v * Dx[tri_index(j, l)];
+                        Kad[ia][jd] += v * Dx[tri_index(j, k)];
+                        Kbc[ib][jc] += v * Dx[tri_index(i, l)];
+                        Kbd[ib][jd] += v * Dx[tri_index(i, k)];
                     }
                 });
             });
+            if (in_tile) {
+                sfor<NA>([&](auto ia_) {
+                    constexpr int ia = decltype(ia_)::value;
+                    if constexpr (chunk_has_a(C0, ia)) {
+                        sfor<NC>([&](auto jc_) {
+                            constexpr int jc = decltype(jc_)::value;
+                            lds_add(&T[ia * tn + oC + jc], Kac[ia][jc]);
+                        });
+                        sfor<ND>([&](auto jd_) {
+                            constexpr int jd = decltype(jd_)::value;
+                            lds_add(&T[ia * tn + oD + jd], Kad[ia][jd]);
+                        });
+                    }
+                });
+                sfor<NB>([&](auto ib_) {
+                    constexpr int ib = decltype(ib_)::value;
+                    if constexpr (chunk_has_b(C0, ib)) {
+                        sfor<NC>([&](auto jc_) {
+                            constexpr int jc = decltype(jc_)::value;
+                            lds_add(&T[(NA + ib) * tn + oC + jc], Kbc[ib][jc]);
+                        });
+                        sfor<ND>([&](auto jd_) {
+                            constexpr int jd = decltype(jd_)::value;
+                            lds_add(&T[(NA + ib) * tn + oD + jd], Kbd[ib][jd]);
+                        });
+                    }
+                });
+                return;
+            }
             sfor<NA>([&](auto ia_) {
                 constexpr int ia = decltype(ia_)::value;
                 if constexpr (chunk_has_a(C0, ia)) {
                     sfor<NC>([&](auto jc_) {
                         constexpr int jc = decltype(jc_)::value;
-                        atomic_add(&AKx[(oA + ia) + (oC + jc) * N], Kac[ia][jc]);
+                        atomic_add(&AKx[tri_index(oA + ia, oC + jc)], Kac[ia][jc]);
                     });
                     sfor<ND>([&](auto jd_) {
                         constexpr int jd = decltype(jd_)::value;
-                        atomic_add(&AKx[(oA + ia) + (oD + jd) * N], Kad[ia][jd]);
+                        atomic_add(&AKx[tri_index(oA + ia, oD + jd)], Kad[ia][jd]);
                     });
                 }
             });
@@ -319,18 +365,55 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                 if constexpr (chunk_has_b(C0, ib)) {
                     sfor<NC>([&](auto jc_) {
                         constexpr int jc = decltype(jc_)::value;
-                        atomic_add(&AKx[(oB + ib) + (oC + jc) * N], Kbc[ib][jc]);
+                        atomic_add(&AKx[tri_index(oB + ib, oC + jc)], Kbc[ib][jc]);
                     });
                     sfor<ND>([&](auto jd_) {
                         constexpr int jd = decltype(jd_)::value;
-                        atomic_add(&AKx[(oB + ib) + (oD + jd) * N], Kbd[ib][jd]);
+                        atomic_add(&AKx[tri_index(oB + ib, oD + jd)], Kbd[ib][jd]);
                     });
                 }
             });
         };
-        exchange(Da, a.AKa + copy + base);
-        if (Db) exchange(Db, a.AKb + copy + base);
+        exchange(Da, a.AKa + copy + base, tile);
+        if (Db) exchange(Db, a.AKb + copy + base, tile ? tile + (NA + NB) * tn : nullptr);
+        if (in_tile && valid) {
+            unsigned int* mask = (unsigned int*)(tile + (Db ? 2 : 1) * (NA + NB) * tn);
+            const int lc = a.sh_local[sc], ld = a.sh_local[sd];
+            // most ket shells are already marked: test before the (contended) LDS atomic
+            if (!((mask[lc >> 5] >> (lc & 31)) & 1u))
+                __hip_atomic_fetch_or(&mask[lc >> 5], 1u << (lc & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!((mask[ld >> 5] >> (ld & 31)) & 1u))
+                __hip_atomic_fetch_or(&mask[ld >> 5], 1u << (ld & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
+}
+
+// Adds the columns of the touched ket shells of the block's LDS exchange rows of bra pair
+// pid to the accumulators and clears them and the mask; called by the whole block.
+__device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles) {
+    const int sa = a.pair_a[pid], sb = a.pair_b[pid];
+    const int tn = a.tile_n;
+    const int oA = a.sh_ao[sa], oB = a.sh_ao[sb];
+    const int g0 = sa - a.sh_local[sa], ns = a.sh_ns[sa];
+    const long long copy = (long long)(xcd_id() % a.n_copies) * a.acc_stride + a.sh_tbase[sa];
+    unsigned int* mask = (unsigned int*)(tile + n_tiles * (NA + NB) * tn);
+    for (int sh = threadIdx.x; sh < ns; sh += blockDim.x) {
+        if (!((mask[sh >> 5] >> (sh & 31)) & 1u)) continue;
+        const int o = a.sh_ao[g0 + sh], nc = a.sh_nao[g0 + sh];
+        for (int t = 0; t < n_tiles; ++t) {
+            double* AK = (t == 0 ? a.AKa : a.AKb) + copy;
+            double* T = tile + t * (NA + NB) * tn;
+            for (int r = 0; r < NA + NB; ++r) {
+                const int row = r < NA ? oA + r : oB + (r - NA);
+                for (int c = 0; c < nc; ++c) {
+                    atomic_add(&AK[tri_index(row, o + c)], T[r * tn + o + c]);
+                    T[r * tn + o + c] = 0.0;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < (ns + 31) / 32; w += blockDim.x) mask[w] = 0u;
 }
 
 // The class index is a template argument only so that profiles name each class's kernels.
@@ -338,20 +421,68 @@ template <int KC, class R, int C0>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     using S = BoysSlice<LTOT, R>;
     __shared__ R tab[BOYS_POINTS * S::nord];
+    extern __shared__ double tile_mem[];
+    __shared__ long long s_chunk;
     const long long lo = a.range[0], hi = a.range[1];
     const long long n = hi - lo;
-    const long long first = (long long)blockIdx.x * blockDim.x;
-    if (first >= n) return;  // the whole block has no work: uniform exit before the barrier
+    if (a.chunk_ctr == nullptr) {
+        const long long first = (long long)blockIdx.x * blockDim.x;
+        if (first >= n) return;  // the whole block has no work: uniform exit before the barrier
+        load_boys<LTOT, R>(boys_of<R>(a), tab);
+        __syncthreads();
+        // Grid-stride over the list; the trip count is block-uniform, so every lane reaches
+        // the wavefront reductions of the digest.
+        const long long stride = (long long)gridDim.x * blockDim.x;
+        for (long long base = first; base < n; base += stride) {
+            const long long w = base + threadIdx.x;
+            const bool valid = w < n;
+            quartet<R, C0>(a, lo + (valid ? w : 0), valid, tab, nullptr, -1);
+        }
+        return;
+    }
+    if (n <= 0) return;
+    unsigned int* ctr = a.chunk_ctr + C0 / CB;
+    const int n_tiles = a.tile_n > 0 ? (a.Db ? 2 : 1) : 0;
+    double* tile = n_tiles ? tile_mem : nullptr;
     load_boys<LTOT, R>(boys_of<R>(a), tab);
-    __syncthreads();
-    // Grid-stride over the list, which is ordered heaviest-first, so every block sees the
-    // same mix of long and short quartets.  The trip count is block-uniform: every lane
-    // reaches the wavefront reductions of the digest.
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = first; base < n; base += stride) {
-        const long long w = base + threadIdx.x;
-        const bool valid = w < n;
-        quartet<R, C0>(a, lo + (valid ? w : 0), valid, tab);
+    for (int x = threadIdx.x; x < n_tiles * (NA + NB) * a.tile_n; x += blockDim.x) tile_mem[x] = 0.0;
+    if (tile)
+        for (int w = threadIdx.x; w < (a.tile_shells + 31) / 32; w += blockDim.x)
+            ((unsigned int*)(tile_mem + n_tiles * (NA + NB) * a.tile_n))[w] = 0u;
+    int tile_pid = -1;
+    // Chunks of at most chunk_len entries, shorter for short lists so every block gets work.
+    long long L = (n + gridDim.x - 1) / gridDim.x;
+    L = ((L + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    if (L > a.chunk_len) L = a.chunk_len;
+    for (;;) {
+        __syncthreads();  // every thread has read s_chunk of the previous round
+        if (threadIdx.x == 0) s_chunk = (long long)atomicAdd(ctr, 1u) * L;
+        __syncthreads();
+        const long long c0 = s_chunk;
+        if (c0 >= n) break;
+        const long long c1 = c0 + L < n ? c0 + L : n;
+        for (long long base = c0; base < c1; base += blockDim.x) {
+            if (tile) {
+                // The rows follow the bra pair of the iteration's first entry once it holds
+                // at least a quarter of the iteration.
+                const int pid0 = (int)a.list[lo + base].x;
+                const long long w = base + threadIdx.x;
+                const int mine = (w < c1 && (int)a.list[lo + w].x == pid0) ? 1 : 0;
+                const int cnt = __syncthreads_count(mine);
+                if (pid0 != tile_pid && cnt >= (int)blockDim.x / 4) {
+                    if (tile_pid >= 0) flush_rows(a, tile, tile_pid, n_tiles);
+                    __syncthreads();
+                    tile_pid = pid0;
+                }
+            }
+            const long long w = base + threadIdx.x;
+            const bool valid = w < c1;
+            quartet<R, C0>(a, lo + (valid ? w : c0), valid, tab, tile, tile_pid);
+        }
+    }
+    if (tile && tile_pid >= 0) {
+        __syncthreads();
+        flush_rows(a, tile, tile_pid, n_tiles);
     }
 }
 
@@ -368,7 +499,10 @@ template <class R>
 void launch_part(const EriArgs& a, int grid, hipStream_t s) {
     sfor_host<(CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0)>([&](auto ch) {
         constexpr int C0 = (CH_BEGIN + decltype(ch)::value) * CB;
-        hipLaunchKernelGGL((k_eri<K_COMBO, R, C0>), dim3(grid), dim3(256), 0, s, a);
+        const size_t lds = a.tile_n > 0 ? (size_t)(a.Db ? 2 : 1) * (NA + NB) * a.tile_n * sizeof(double) +
+                                              (size_t)((a.tile_shells + 31) / 32) * sizeof(unsigned int)
+                                        : 0;
+        hipLaunchKernelGGL((k_eri<K_COMBO, R, C0>), dim3(grid), dim3(256), lds, s, a);
     });
 }
 

@@ -9,29 +9,14 @@ import numpy as np
 _hip = None
 
 
-def _mapped_hip():
-    """Path of the HIP runtime already mapped into this process, if any."""
-    try:
-        with open("/proc/self/maps") as f:
-            for line in f:
-                if "libamdhip64" in line:
-                    return line.split()[-1]
-    except OSError:
-        pass
-    return None
-
-
 def _lib():
     global _hip
     if _hip is None:
-        # Reuse the HIP runtime the engine library runs on (torch's, when torch was imported
-        # first): a second runtime in one process does not see the device.
-        path = _mapped_hip()
-        if path is None:
-            from . import lib
-            lib.load()
-            path = _mapped_hip() or "libamdhip64.so.7"
-        _hip = C.CDLL(path)
+        # The runtime functions are looked up through the engine library's handle (dlsym
+        # searches the library's own dependencies), so these buffers live in the same HIP
+        # runtime instance as the engine, whichever other copies (torch's) the process maps.
+        from . import lib
+        _hip = lib.load().dll
         _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         _hip.hipFree.argtypes = [C.c_void_p]
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
