@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -97,9 +98,9 @@ int eri_chunks(int combo) {
     return n > 0 ? n : -1;
 }
 
-void launch_eri(int combo, int fp32, const EriArgs& a, int grid, hipStream_t s) {
+void launch_eri(int combo, int fp32, const EriArgs& a, int grid, const hipStream_t* chunk_streams) {
     PartTable& t = parts();
-    for (int k = 0; k < t.nparts[combo]; ++k) t.fn[combo][k](fp32, a, grid, s);
+    for (int k = 0; k < t.nparts[combo]; ++k) t.fn[combo][k](fp32, a, grid, chunk_streams);
 }
 
 }  // namespace hfx
@@ -597,6 +598,12 @@ struct hfx_engine {
     int device = 0;
     int n_cu = 256;
     hipStream_t stream = nullptr;
+    // Streams the class kernels of a build are spread over (side[0] is `stream`); the
+    // captured graph has one branch per stream.
+    static constexpr int N_SIDE = 4;
+    int n_side = N_SIDE;  // streams in use (HFX_BUILD_STREAMS, 1..N_SIDE)
+    hipStream_t side[N_SIDE] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[N_SIDE] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevArena mem;
 
@@ -694,6 +701,11 @@ struct hfx_engine {
             if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (int i = 1; i < N_SIDE; ++i) {
+            if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -851,7 +863,8 @@ int store_blocks(hfx_engine* e, int combo, const std::vector<uint2>& ents, std::
     a.range = drange;
     a.out = dout;
     a.out_off = doff;
-    launch_eri(combo, 0, a, eri_grid(e), e->stream);
+    std::vector<hipStream_t> cs(std::max(1, eri_chunks(combo)), e->stream);
+    launch_eri(combo, 0, a, eri_grid(e), cs.data());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(out.data(), dout, n * blk * sizeof(double), hipMemcpyDeviceToHost));
@@ -907,6 +920,13 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     e->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->side[0] = e->stream;
+    for (int i = 1; i < hfx_engine::N_SIDE; ++i) {
+        HIPCHK(hipStreamCreateWithFlags(&e->side[i], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&e->ev_join[i], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    if (const char* v = std::getenv("HFX_BUILD_STREAMS")) e->n_side = std::max(1, std::min(hfx_engine::N_SIDE, std::atoi(v)));
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
     for (hipEvent_t& x : e->cev) HIPCHK(hipEventCreate(&x));
@@ -1232,6 +1252,30 @@ int enqueue_screen(hfx_engine* e, const BuildParams& P, hipStream_t s) {
     return HFX_OK;
 }
 
+// Static cost (arbitrary units) of one chunk kernel of class k in precision fp32 (1) or
+// FP64 (0): the unscreened model FLOPs of the class at its mean unpruned primitive counts,
+// split over the class's chunk kernels, plus a fixed launch cost.  In mixed mode most
+// quartets run in FP32; the FP64 share is small but runs on an under-filled machine.
+double chunk_cost(hfx_engine* e, int k, int fp32, bool mixed) {
+    const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+    const std::vector<int>& b = e->cls_ids[p1];
+    const std::vector<int>& c = e->cls_ids[p2];
+    if (b.empty() || c.empty()) return 0.0;
+    auto mean_np = [&](const std::vector<int>& ids) {
+        double t = 0.0;
+        for (int id : ids) t += e->pair_nprim[id];
+        return t / (double)ids.size();
+    };
+    const double nb = (double)b.size(), nc = (double)c.size();
+    const double quartets = p1 == p2 ? 0.5 * nb * (nb + 1.0) : nb * nc;
+    const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
+    const double per = mean_np(b) * mean_np(c) * model_flop_h(la, lb, lc, ld) +
+                       12.0 * ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+    const double share = !mixed ? 1.0 : (fp32 ? 0.9 : 0.25);
+    const int nch = std::max(1, eri_chunks(k));
+    return 1e-9 * quartets * per * share / nch + 1.0;
+}
+
 int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_slot, int& n_ev) {
     const bool uhf = P.uhf, mixed = P.mixed, timing = P.timing;
     const long long ne = e->n_elems;
@@ -1281,6 +1325,38 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         const size_t mask = (size_t)((e->ns_max + 31) / 32) * sizeof(unsigned int);
         return boys + rows + mask + 1024 <= 64 * 1024 ? e->max_N : 0;
     };
+    // Chunk kernels go to the streams by longest-processing-time-first on a static cost
+    // estimate; with per-class timing everything stays on the build stream, in class order.
+    const int n_str = timing ? 1 : e->n_side;
+    std::vector<hipStream_t> cs[2 * N_COMBOS];
+    {
+        struct Item {
+            double cost;
+            int slot, ch;
+        };
+        std::vector<Item> items;
+        for (int k = 0; k < N_COMBOS; ++k) {
+            const int nch = eri_chunks(k);
+            for (int f = 0; f < (mixed ? 2 : 1); ++f) {
+                cs[2 * k + f].assign(std::max(1, nch), s);
+                const double c = chunk_cost(e, k, f, mixed);
+                for (int ch = 0; ch < nch; ++ch) items.push_back({c, 2 * k + f, ch});
+            }
+        }
+        std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+        double load[hfx_engine::N_SIDE] = {};
+        for (const Item& it : items) {
+            int best = 0;
+            for (int i = 1; i < n_str; ++i)
+                if (load[i] < load[best]) best = i;
+            load[best] += it.cost;
+            cs[it.slot][it.ch] = e->side[best];
+        }
+    }
+    if (n_str > 1) {
+        HIPCHK(hipEventRecord(e->ev_fork, s));
+        for (int i = 1; i < n_str; ++i) HIPCHK(hipStreamWaitEvent(e->side[i], e->ev_fork, 0));
+    }
     if (timing) HIPCHK(hipEventRecord(e->cev[n_ev++], s));
     for (int k = 0; k < N_COMBOS; ++k) {
         ev_slot[2 * k] = ev_slot[2 * k + 1] = -1;
@@ -1289,7 +1365,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         ea.range = e->d_ranges + 2 * k;
         ea.chunk_ctr = e->d_chunk_ctr + (2 * k) * hfx_engine::CHUNK_SLOTS;
         ea.tile_n = tile_cols(k, 0);
-        launch_eri(k, 0, ea, grid, s);
+        launch_eri(k, 0, ea, grid, cs[2 * k].data());
         if (timing) {
             HIPCHK(hipEventRecord(e->cev[n_ev], s));
             ev_slot[2 * k] = n_ev++;
@@ -1298,11 +1374,17 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
             ea.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
             ea.chunk_ctr = e->d_chunk_ctr + (2 * k + 1) * hfx_engine::CHUNK_SLOTS;
             ea.tile_n = tile_cols(k, 1);
-            launch_eri(k, 1, ea, grid, s);
+            launch_eri(k, 1, ea, grid, cs[2 * k + 1].data());
             if (timing) {
                 HIPCHK(hipEventRecord(e->cev[n_ev], s));
                 ev_slot[2 * k + 1] = n_ev++;
             }
+        }
+    }
+    if (n_str > 1) {
+        for (int i = 1; i < n_str; ++i) {
+            HIPCHK(hipEventRecord(e->ev_join[i], e->side[i]));
+            HIPCHK(hipStreamWaitEvent(s, e->ev_join[i], 0));
         }
     }
     {

@@ -496,21 +496,21 @@ constexpr int CH_BEGIN = HFX_CHUNK_BEGIN;
 constexpr int CH_END = HFX_CHUNK_END < NCH ? HFX_CHUNK_END : NCH;
 
 template <class R>
-void launch_part(const EriArgs& a, int grid, hipStream_t s) {
+void launch_part(const EriArgs& a, int grid, const hipStream_t* cs) {
     sfor_host<(CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0)>([&](auto ch) {
         constexpr int C0 = (CH_BEGIN + decltype(ch)::value) * CB;
         const size_t lds = a.tile_n > 0 ? (size_t)(a.Db ? 2 : 1) * (NA + NB) * a.tile_n * sizeof(double) +
                                               (size_t)((a.tile_shells + 31) / 32) * sizeof(unsigned int)
                                         : 0;
-        hipLaunchKernelGGL((k_eri<K_COMBO, R, C0>), dim3(grid), dim3(256), lds, s, a);
+        hipLaunchKernelGGL((k_eri<K_COMBO, R, C0>), dim3(grid), dim3(256), lds, cs[C0 / CB], a);
     });
 }
 
-void launch_this_part(int fp32, const EriArgs& a, int grid, hipStream_t s) {
+void launch_this_part(int fp32, const EriArgs& a, int grid, const hipStream_t* cs) {
     if (fp32)
-        launch_part<float>(a, grid, s);
+        launch_part<float>(a, grid, cs);
     else
-        launch_part<double>(a, grid, s);
+        launch_part<double>(a, grid, cs);
 }
 
 const bool registered = register_eri_part(K_COMBO, CH_BEGIN, CH_END, NCH, &launch_this_part);
