@@ -98,6 +98,15 @@ int eri_chunks(int combo) {
     return n > 0 ? n : -1;
 }
 
+int eri_parts(int combo, int* begin, int* end) {
+    PartTable& t = parts();
+    for (int k = 0; k < t.nparts[combo]; ++k) {
+        begin[k] = t.begin[combo][k];
+        end[k] = std::min(t.end[combo][k], t.nch[combo]);
+    }
+    return t.nparts[combo];
+}
+
 void launch_eri(int combo, int fp32, const EriArgs& a, int grid, const hipStream_t* chunk_streams) {
     PartTable& t = parts();
     for (int k = 0; k < t.nparts[combo]; ++k) t.fn[combo][k](fp32, a, grid, chunk_streams);
@@ -1330,17 +1339,21 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     const int n_str = timing ? 1 : e->n_side;
     std::vector<hipStream_t> cs[2 * N_COMBOS];
     {
+        // one item per launch: the chunks [b, e) of one translation unit of a class
         struct Item {
             double cost;
-            int slot, ch;
+            int slot, b, e;
         };
         std::vector<Item> items;
+        int pb[64], pe[64];
         for (int k = 0; k < N_COMBOS; ++k) {
             const int nch = eri_chunks(k);
+            const int np = eri_parts(k, pb, pe);
             for (int f = 0; f < (mixed ? 2 : 1); ++f) {
                 cs[2 * k + f].assign(std::max(1, nch), s);
                 const double c = chunk_cost(e, k, f, mixed);
-                for (int ch = 0; ch < nch; ++ch) items.push_back({c, 2 * k + f, ch});
+                for (int x = 0; x < np; ++x)
+                    if (pe[x] > pb[x]) items.push_back({c * (pe[x] - pb[x]), 2 * k + f, pb[x], pe[x]});
             }
         }
         std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
@@ -1350,7 +1363,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
             for (int i = 1; i < n_str; ++i)
                 if (load[i] < load[best]) best = i;
             load[best] += it.cost;
-            cs[it.slot][it.ch] = e->side[best];
+            for (int ch = it.b; ch < it.e; ++ch) cs[it.slot][ch] = e->side[best];
         }
     }
     if (n_str > 1) {

@@ -55,6 +55,8 @@ constexpr int CB = (NAB + NCH - 1) / NCH;
 constexpr int LTOT = LA + LB + LC + LD;
 constexpr int LCD = LC + LD;
 constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
+// Light classes take two ket primitive pairs per bra load.
+constexpr bool KET_PAIRS = LTOT <= 2;
 
 template <class R>
 __device__ __forceinline__ const PrimPair<R>* pp_of(const EriArgs& a);
@@ -109,10 +111,104 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
     R acc[CB][NCD];
     sfor<CB>([&](auto c) { sfor<NCD>([&](auto d) { acc[c][d] = R(0); }); });
 
+    // One primitive quartet: bra primitive pair A (with its E coefficients), ket B (with F).
+    auto prim = [&](const PrimPair<R>& A, const Ecoef<LA, LB, R>& Ex, const Ecoef<LA, LB, R>& Ey,
+                    const Ecoef<LA, LB, R>& Ez, const PrimPair<R>& B, const Ecoef<LC, LD, R>& Fx,
+                    const Ecoef<LC, LD, R>& Fy, const Ecoef<LC, LD, R>& Fz) {
+        const R pq = A.p + B.p;
+        const R rs = dev_rsqrt(pq);
+        const R alpha = (A.p * B.p) * (rs * rs);
+        const R X = A.px - B.px, Y = A.py - B.py, Z = A.pz - B.pz;
+        const R T = alpha * (X * X + Y * Y + Z * Z);
+        const R pref = R(TWO_PI_2_5) * rs * A.kp * B.kp;
+        R F[LTOT + 1];
+        boys<LTOT>(T, tab, F);
+        R base[LTOT + 1];
+        {
+            const R m2a = R(-2) * alpha;
+            R pw = pref;
+            sfor<LTOT + 1>([&](auto n) {
+                base[n] = pw * F[n];
+                if constexpr (decltype(n)::value < LTOT) pw *= m2a;
+            });
+        }
+        R Rt[nherm(LTOT)];
+        hermite_r<LTOT>(base, X, Y, Z, Rt);
+
+        sfor<CB>([&](auto c_) {
+            constexpr int c = decltype(c_)::value;
+            constexpr int ab = C0 + c;
+            if constexpr (ab < NAB) {
+                constexpr int ia = ab / NB, ib = ab % NB;
+                constexpr int ax = cart_i(LA, ia), ay = cart_j(LA, ia), az = cart_k(LA, ia);
+                constexpr int bx = cart_i(LB, ib), by = cart_j(LB, ib), bz = cart_k(LB, ib);
+                R Yv[nherm(LCD)];
+                sfor<nherm(LCD)>([&](auto h_) {
+                    constexpr int h = decltype(h_)::value;
+                    constexpr int t2 = h_t(h), u2 = h_u(h), v2 = h_v(h);
+                    Yv[h] = sum_of<az + bz + 1>([&](auto v_) {
+                        constexpr int v = decltype(v_)::value;
+                        return Ez.e[az][bz][v] * sum_of<ay + by + 1>([&](auto u_) {
+                                   constexpr int u = decltype(u_)::value;
+                                   return Ey.e[ay][by][u] * sum_of<ax + bx + 1>([&](auto t_) {
+                                              constexpr int t = decltype(t_)::value;
+                                              return Ex.e[ax][bx][t] * Rt[hidx(t + t2, u + u2, v + v2)];
+                                          });
+                               });
+                    });
+                });
+                sfor<NCD>([&](auto cd_) {
+                    constexpr int cd = decltype(cd_)::value;
+                    constexpr int jc = cd / ND, jd = cd % ND;
+                    constexpr int cx = cart_i(LC, jc), cy = cart_j(LC, jc), cz = cart_k(LC, jc);
+                    constexpr int dx = cart_i(LD, jd), dy = cart_j(LD, jd), dz = cart_k(LD, jd);
+                    acc[c][cd] += sum_of<cz + dz + 1>([&](auto v_) {
+                        constexpr int v = decltype(v_)::value;
+                        return Fz.e[cz][dz][v] * sum_of<cy + dy + 1>([&](auto u_) {
+                                   constexpr int u = decltype(u_)::value;
+                                   return Fy.e[cy][dy][u] * sum_of<cx + dx + 1>([&](auto t_) {
+                                              constexpr int t = decltype(t_)::value;
+                                              constexpr R sg = ((t + u + v) & 1) ? R(-1) : R(1);
+                                              return (sg * Fx.e[cx][dx][t]) * Yv[hidx(t, u, v)];
+                                          });
+                               });
+                    });
+                });
+            }
+        });
+    };
+
     // Outer loop over the ket's primitive pairs (private to the lane), inner loop over the
     // bra's: the work list groups quartets by bra pair, so the inner loads are shared by the
-    // lanes of a wavefront.
-    for (int iq = 0; iq < nq; ++iq) {
+    // lanes of a wavefront.  For the light classes the outer loop takes two ket primitive
+    // pairs at a time, so each bra load serves two independent primitive quartets; an odd
+    // last one gets a pass of its own.
+    int iq = 0;
+    if constexpr (KET_PAIRS) {
+        for (; iq + 1 < nq; iq += 2) {
+            const PrimPair<R> B0 = pp[qs + iq], B1 = pp[qs + iq + 1];
+            Ecoef<LC, LD, R> Fx0, Fy0, Fz0, Fx1, Fy1, Fz1;
+            hermite_e<LC, LD>(B0.oo2p, B0.pax, B0.pbx, Fx0);
+            hermite_e<LC, LD>(B0.oo2p, B0.pay, B0.pby, Fy0);
+            hermite_e<LC, LD>(B0.oo2p, B0.paz, B0.pbz, Fz0);
+            hermite_e<LC, LD>(B1.oo2p, B1.pax, B1.pbx, Fx1);
+            hermite_e<LC, LD>(B1.oo2p, B1.pay, B1.pby, Fy1);
+            hermite_e<LC, LD>(B1.oo2p, B1.paz, B1.pbz, Fz1);
+            PrimPair<R> A;
+            if (np > 0) A = pp[ps];
+            for (int ip = 0; ip < np; ++ip) {
+                const PrimPair<R> An = pp[ps + (ip + 1 < np ? ip + 1 : ip)];
+                Ecoef<LA, LB, R> Ex, Ey, Ez;
+                hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
+                hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
+                hermite_e<LA, LB>(A.oo2p, A.paz, A.pbz, Ez);
+                prim(A, Ex, Ey, Ez, B0, Fx0, Fy0, Fz0);
+                prim(A, Ex, Ey, Ez, B1, Fx1, Fy1, Fz1);
+                A = An;
+            }
+        }
+    }
+    for (; iq < nq; ++iq) {
         const PrimPair<R> B = pp[qs + iq];
         Ecoef<LC, LD, R> Fx, Fy, Fz;
         hermite_e<LC, LD>(B.oo2p, B.pax, B.pbx, Fx);
@@ -127,67 +223,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
             hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
             hermite_e<LA, LB>(A.oo2p, A.paz, A.pbz, Ez);
-            const R pq = A.p + B.p;
-            const R rs = dev_rsqrt(pq);
-            const R alpha = (A.p * B.p) * (rs * rs);
-            const R X = A.px - B.px, Y = A.py - B.py, Z = A.pz - B.pz;
-            const R T = alpha * (X * X + Y * Y + Z * Z);
-            const R pref = R(TWO_PI_2_5) * rs * A.kp * B.kp;
-            R F[LTOT + 1];
-            boys<LTOT>(T, tab, F);
-            R base[LTOT + 1];
-            {
-                const R m2a = R(-2) * alpha;
-                R pw = pref;
-                sfor<LTOT + 1>([&](auto n) {
-                    base[n] = pw * F[n];
-                    if constexpr (decltype(n)::value < LTOT) pw *= m2a;
-                });
-            }
-            R Rt[nherm(LTOT)];
-            hermite_r<LTOT>(base, X, Y, Z, Rt);
-
-            sfor<CB>([&](auto c_) {
-                constexpr int c = decltype(c_)::value;
-                constexpr int ab = C0 + c;
-                if constexpr (ab < NAB) {
-                    constexpr int ia = ab / NB, ib = ab % NB;
-                    constexpr int ax = cart_i(LA, ia), ay = cart_j(LA, ia), az = cart_k(LA, ia);
-                    constexpr int bx = cart_i(LB, ib), by = cart_j(LB, ib), bz = cart_k(LB, ib);
-                    R Yv[nherm(LCD)];
-                    sfor<nherm(LCD)>([&](auto h_) {
-                        constexpr int h = decltype(h_)::value;
-                        constexpr int t2 = h_t(h), u2 = h_u(h), v2 = h_v(h);
-                        Yv[h] = sum_of<az + bz + 1>([&](auto v_) {
-                            constexpr int v = decltype(v_)::value;
-                            return Ez.e[az][bz][v] * sum_of<ay + by + 1>([&](auto u_) {
-                                       constexpr int u = decltype(u_)::value;
-                                       return Ey.e[ay][by][u] * sum_of<ax + bx + 1>([&](auto t_) {
-                                                  constexpr int t = decltype(t_)::value;
-                                                  return Ex.e[ax][bx][t] * Rt[hidx(t + t2, u + u2, v + v2)];
-                                              });
-                                   });
-                        });
-                    });
-                    sfor<NCD>([&](auto cd_) {
-                        constexpr int cd = decltype(cd_)::value;
-                        constexpr int jc = cd / ND, jd = cd % ND;
-                        constexpr int cx = cart_i(LC, jc), cy = cart_j(LC, jc), cz = cart_k(LC, jc);
-                        constexpr int dx = cart_i(LD, jd), dy = cart_j(LD, jd), dz = cart_k(LD, jd);
-                        acc[c][cd] += sum_of<cz + dz + 1>([&](auto v_) {
-                            constexpr int v = decltype(v_)::value;
-                            return Fz.e[cz][dz][v] * sum_of<cy + dy + 1>([&](auto u_) {
-                                       constexpr int u = decltype(u_)::value;
-                                       return Fy.e[cy][dy][u] * sum_of<cx + dx + 1>([&](auto t_) {
-                                                  constexpr int t = decltype(t_)::value;
-                                                  constexpr R sg = ((t + u + v) & 1) ? R(-1) : R(1);
-                                                  return (sg * Fx.e[cx][dx][t]) * Yv[hidx(t, u, v)];
-                                              });
-                                   });
-                        });
-                    });
-                }
-            });
+            prim(A, Ex, Ey, Ez, B, Fx, Fy, Fz);
             A = An;
         }
     }
@@ -416,23 +452,20 @@ __device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles)
     for (int w = threadIdx.x; w < (ns + 31) / 32; w += blockDim.x) mask[w] = 0u;
 }
 
-// The class index is a template argument only so that profiles name each class's kernels.
-template <int KC, class R, int C0>
-__global__ __launch_bounds__(256) void k_eri(EriArgs a) {
-    using S = BoysSlice<LTOT, R>;
-    __shared__ R tab[BOYS_POINTS * S::nord];
-    extern __shared__ double tile_mem[];
-    __shared__ long long s_chunk;
+// The work of one chunk kernel on virtual block vb of nvb (several chunks share a launch).
+template <class R, int C0>
+__device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* tab, double* tile_mem,
+                                          long long& s_chunk) {
     const long long lo = a.range[0], hi = a.range[1];
     const long long n = hi - lo;
     if (a.chunk_ctr == nullptr) {
-        const long long first = (long long)blockIdx.x * blockDim.x;
+        const long long first = (long long)vb * blockDim.x;
         if (first >= n) return;  // the whole block has no work: uniform exit before the barrier
         load_boys<LTOT, R>(boys_of<R>(a), tab);
         __syncthreads();
         // Grid-stride over the list; the trip count is block-uniform, so every lane reaches
         // the wavefront reductions of the digest.
-        const long long stride = (long long)gridDim.x * blockDim.x;
+        const long long stride = (long long)nvb * blockDim.x;
         for (long long base = first; base < n; base += stride) {
             const long long w = base + threadIdx.x;
             const bool valid = w < n;
@@ -451,7 +484,7 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
             ((unsigned int*)(tile_mem + n_tiles * (NA + NB) * a.tile_n))[w] = 0u;
     int tile_pid = -1;
     // Chunks of at most chunk_len entries, shorter for short lists so every block gets work.
-    long long L = (n + gridDim.x - 1) / gridDim.x;
+    long long L = (n + nvb - 1) / nvb;
     L = ((L + blockDim.x - 1) / blockDim.x) * blockDim.x;
     if (L > a.chunk_len) L = a.chunk_len;
     for (;;) {
@@ -486,6 +519,22 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     }
 }
 
+// Chunks [CH0, CH0 + NC) of the class in one launch: block b works on chunk b % NC as
+// virtual block b / NC, so the chunks are interleaved over the machine and run side by side.
+// The class index is a template argument only so that profiles name each class's kernels.
+template <int KC, class R, int CH0, int NC>
+__global__ __launch_bounds__(256) void k_eri(EriArgs a) {
+    using S = BoysSlice<LTOT, R>;
+    __shared__ R tab[BOYS_POINTS * S::nord];
+    __shared__ long long s_chunk;
+    extern __shared__ double tile_mem[];
+    const int ch = (int)(blockIdx.x % NC), vb = (int)(blockIdx.x / NC), nvb = (int)(gridDim.x / NC);
+    sfor<NC>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        if (ch == c) run_chunk<R, (CH0 + c) * CB>(a, vb, nvb, tab, tile_mem, s_chunk);
+    });
+}
+
 #ifndef HFX_CHUNK_BEGIN
 #define HFX_CHUNK_BEGIN 0
 #endif
@@ -497,13 +546,14 @@ constexpr int CH_END = HFX_CHUNK_END < NCH ? HFX_CHUNK_END : NCH;
 
 template <class R>
 void launch_part(const EriArgs& a, int grid, const hipStream_t* cs) {
-    sfor_host<(CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0)>([&](auto ch) {
-        constexpr int C0 = (CH_BEGIN + decltype(ch)::value) * CB;
+    constexpr int NC = CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0;
+    if constexpr (NC > 0) {
         const size_t lds = a.tile_n > 0 ? (size_t)(a.Db ? 2 : 1) * (NA + NB) * a.tile_n * sizeof(double) +
                                               (size_t)((a.tile_shells + 31) / 32) * sizeof(unsigned int)
                                         : 0;
-        hipLaunchKernelGGL((k_eri<K_COMBO, R, C0>), dim3(grid), dim3(256), lds, cs[C0 / CB], a);
-    });
+        const int blocks = (grid > NC ? grid / NC : 1) * NC;
+        hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC>), dim3(blocks), dim3(256), lds, cs[CH_BEGIN], a);
+    }
 }
 
 void launch_this_part(int fp32, const EriArgs& a, int grid, const hipStream_t* cs) {
