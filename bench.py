@@ -16,7 +16,7 @@ build (one hfx_jk call, dD = D_conv, D_prev = 0) from device-resident inputs.  T
      of the same build, scaled by the model FLOPs of the slice
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Profiling aid: --save-density PATH stores the converged FP64 density; a later
+Profiling aid: --save-density PATH stores the converged FP64 density (and stops); a later
   python bench.py --profile-builds --density PATH --cut C --steps K --warmup W
 runs only the W + K mixed builds with per-class HIP-event timing (no SCF), so a profiler's
 per-kernel averages cover exactly the launches the roofline is computed from.
@@ -227,6 +227,8 @@ def main():
     Da = r64["D_alpha"]
     if args.save_density:
         np.save(args.save_density, Da)
+        log("saved the converged FP64 density to %s" % args.save_density)
+        return
 
     eng = lib.Engine(L, basis)
     Q = eng.schwarz()

@@ -119,7 +119,7 @@ namespace {
 constexpr int CMAX = 6;  // Cartesian components of a d shell
 
 // ---------------------------------------------------------------------------------------
-// Boys table: F_m(i * 0.1), i = 0..300, m = 0..16, from the series
+// Boys table: F_m(i * 0.1), i = 0..300, m = 0..16, and e^{-i * 0.1} in the last column, from the series
 // F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double.
 // ---------------------------------------------------------------------------------------
 long double boys_series_ld(int m, long double T) {
@@ -1082,10 +1082,12 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     for (int i = 0; i < md::BOYS_POINTS; ++i) {
         const double T64 = i * 0.1;
         const float T32 = (float)i * 0.1f;
-        for (int m = 0; m < md::BOYS_ORDERS; ++m) {
+        for (int m = 0; m < md::BOYS_EXP_COL; ++m) {
             b64[(size_t)i * md::BOYS_ORDERS + m] = (double)boys_series_ld(m, (long double)T64);
             b32[(size_t)i * md::BOYS_ORDERS + m] = (float)boys_series_ld(m, (long double)T32);
         }
+        b64[(size_t)i * md::BOYS_ORDERS + md::BOYS_EXP_COL] = (double)std::exp(-(long double)T64);
+        b32[(size_t)i * md::BOYS_ORDERS + md::BOYS_EXP_COL] = (float)std::exp(-(long double)T32);
     }
 
     // ---- work-list regions: unscreened canonical quartets per class ----
@@ -1328,7 +1330,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
         if (L == 0) return 0;
-        const int nt = fp32 ? 5 : 8, nord = L <= 1 ? L + nt : nt;
+        const int nt = fp32 ? 5 : 8, nord = L <= 1 ? L + nt : nt + (fp32 ? 0 : 1);
         const size_t boys = (size_t)md::BOYS_POINTS * nord * (fp32 ? 4 : 8);
         const size_t rows = (size_t)(uhf ? 2 : 1) * (ncart_h(la) + ncart_h(lb)) * e->max_N * sizeof(double);
         const size_t mask = (size_t)((e->ns_max + 31) / 32) * sizeof(unsigned int);

@@ -326,15 +326,19 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
         // LDS rows T instead when the block holds this bra pair).
         const bool in_tile = tile != nullptr && pid == tile_pid;
         const int tn = a.tile_n;
-        auto exchange = [&](const double* __restrict__ Dx, double* AKx, double* T) {
-            double Kac[NA][NC], Kad[NA][ND], Kbc[NB][NC], Kbd[NB][ND];
-            sfor<NA>([&](auto x) {
-                sfor<NC>([&](auto y) { Kac[x][y] = 0.0; });
-                sfor<ND>([&](auto y) { Kad[x][y] = 0.0; });
-            });
-            sfor<NB>([&](auto x) {
-                sfor<NC>([&](auto y) { Kbc[x][y] = 0.0; });
-                sfor<ND>([&](auto y) { Kbd[x][y] = 0.0; });
+        // The exchange terms of the rows of one bra shell (SIDE 0: rows i of a, paired with
+        // density row j of b; SIDE 1: rows j of b with density row i of a), with one spin
+        // density Dx into the accumulator AKx, or into the LDS rows T when the block holds
+        // this bra pair.  The two sides run one after the other, each with half of the
+        // accumulators and density loads live, which keeps the digest's register peak (the
+        // kernel's allocation) close to that of the primitive loop.
+        auto exchange_side = [&](auto side_, const double* __restrict__ Dx, double* AKx, double* T) {
+            constexpr int SIDE = decltype(side_)::value;
+            constexpr int NR = SIDE == 0 ? NA : NB;  // rows of this side's shell
+            double Kc[NR][NC], Kd[NR][ND];
+            sfor<NR>([&](auto x) {
+                sfor<NC>([&](auto y) { Kc[x][y] = 0.0; });
+                sfor<ND>([&](auto y) { Kd[x][y] = 0.0; });
             });
             sfor<NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
@@ -345,70 +349,47 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                     constexpr int ab = C0 + c;
                     if constexpr (ab < NAB) {
                         constexpr int ia = ab / NB, ib = ab % NB;
-                        const long long i = oA + ia, j = oB + ib;
+                        constexpr int r = SIDE == 0 ? ia : ib;
+                        const long long o = SIDE == 0 ? oB + ib : oA + ia;  // the other bra index
                         const double v = val(c_, cd_);
-                        Kac[ia][jc] += v * Dx[tri_index(j, l)];
-                        Kad[ia][jd] += v * Dx[tri_index(j, k)];
-                        Kbc[ib][jc] += v * Dx[tri_index(i, l)];
-                        Kbd[ib][jd] += v * Dx[tri_index(i, k)];
+                        Kc[r][jc] += v * Dx[tri_index(o, l)];
+                        Kd[r][jd] += v * Dx[tri_index(o, k)];
                     }
                 });
             });
-            if (in_tile) {
-                sfor<NA>([&](auto ia_) {
-                    constexpr int ia = decltype(ia_)::value;
-                    if constexpr (chunk_has_a(C0, ia)) {
+            const int orow = SIDE == 0 ? oA : oB;
+            sfor<NR>([&](auto r_) {
+                constexpr int r = decltype(r_)::value;
+                constexpr bool has = SIDE == 0 ? chunk_has_a(C0, r) : chunk_has_b(C0, r);
+                if constexpr (has) {
+                    if (in_tile) {
                         sfor<NC>([&](auto jc_) {
                             constexpr int jc = decltype(jc_)::value;
-                            lds_add(&T[ia * tn + oC + jc], Kac[ia][jc]);
+                            lds_add(&T[(SIDE * NA + r) * tn + oC + jc], Kc[r][jc]);
                         });
                         sfor<ND>([&](auto jd_) {
                             constexpr int jd = decltype(jd_)::value;
-                            lds_add(&T[ia * tn + oD + jd], Kad[ia][jd]);
+                            lds_add(&T[(SIDE * NA + r) * tn + oD + jd], Kd[r][jd]);
                         });
-                    }
-                });
-                sfor<NB>([&](auto ib_) {
-                    constexpr int ib = decltype(ib_)::value;
-                    if constexpr (chunk_has_b(C0, ib)) {
+                    } else {
                         sfor<NC>([&](auto jc_) {
                             constexpr int jc = decltype(jc_)::value;
-                            lds_add(&T[(NA + ib) * tn + oC + jc], Kbc[ib][jc]);
+                            atomic_add(&AKx[tri_index(orow + r, oC + jc)], Kc[r][jc]);
                         });
                         sfor<ND>([&](auto jd_) {
                             constexpr int jd = decltype(jd_)::value;
-                            lds_add(&T[(NA + ib) * tn + oD + jd], Kbd[ib][jd]);
+                            atomic_add(&AKx[tri_index(orow + r, oD + jd)], Kd[r][jd]);
                         });
                     }
-                });
-                return;
-            }
-            sfor<NA>([&](auto ia_) {
-                constexpr int ia = decltype(ia_)::value;
-                if constexpr (chunk_has_a(C0, ia)) {
-                    sfor<NC>([&](auto jc_) {
-                        constexpr int jc = decltype(jc_)::value;
-                        atomic_add(&AKx[tri_index(oA + ia, oC + jc)], Kac[ia][jc]);
-                    });
-                    sfor<ND>([&](auto jd_) {
-                        constexpr int jd = decltype(jd_)::value;
-                        atomic_add(&AKx[tri_index(oA + ia, oD + jd)], Kad[ia][jd]);
-                    });
                 }
             });
-            sfor<NB>([&](auto ib_) {
-                constexpr int ib = decltype(ib_)::value;
-                if constexpr (chunk_has_b(C0, ib)) {
-                    sfor<NC>([&](auto jc_) {
-                        constexpr int jc = decltype(jc_)::value;
-                        atomic_add(&AKx[tri_index(oB + ib, oC + jc)], Kbc[ib][jc]);
-                    });
-                    sfor<ND>([&](auto jd_) {
-                        constexpr int jd = decltype(jd_)::value;
-                        atomic_add(&AKx[tri_index(oB + ib, oD + jd)], Kbd[ib][jd]);
-                    });
-                }
-            });
+        };
+        auto exchange = [&](const double* __restrict__ Dx, double* AKx, double* T) {
+            __builtin_amdgcn_sched_barrier(0);
+            exchange_side(md::ic<0>{}, Dx, AKx, T);
+            __builtin_amdgcn_sched_barrier(0);
+            exchange_side(md::ic<1>{}, Dx, AKx, T);
+            __builtin_amdgcn_sched_barrier(0);
         };
         exchange(Da, a.AKa + copy + base, tile);
         if (Db) exchange(Db, a.AKb + copy + base, tile ? tile + (NA + NB) * tn : nullptr);

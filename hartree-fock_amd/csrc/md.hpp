@@ -169,7 +169,8 @@ __device__ __forceinline__ void hermite_r(const R (&base)[L + 1], R X, R Y, R Z,
 // A kernel whose highest order is L copies the orders [first, first+nord) into LDS.
 // ---------------------------------------------------------------------------------------
 constexpr int BOYS_POINTS = 301;
-constexpr int BOYS_ORDERS = 17;
+constexpr int BOYS_ORDERS = 18;   // columns per grid point: F_0 .. F_16, then e^{-T_i}
+constexpr int BOYS_EXP_COL = 17;
 
 template <class R>
 struct BoysTerms;
@@ -186,7 +187,9 @@ template <int L, class R>
 struct BoysSlice {
     static constexpr int nt = BoysTerms<R>::nt;
     static constexpr int first = (L <= 1) ? 0 : L;
-    static constexpr int nord = (L <= 1) ? L + nt : nt;
+    // An FP64 slice used with the downward recursion also carries e^{-T_i} (its last column).
+    static constexpr bool texp = L >= 2 && sizeof(R) == 8;
+    static constexpr int nord = (L <= 1) ? L + nt : nt + (texp ? 1 : 0);
 };
 
 template <class R>
@@ -221,7 +224,8 @@ __device__ __forceinline__ void load_boys(const R* __restrict__ g, R* lds) {
     using S = BoysSlice<L, R>;
     for (int x = threadIdx.x; x < BOYS_POINTS * S::nord; x += blockDim.x) {
         const int i = x / S::nord, m = x - i * S::nord;
-        lds[x] = g[i * BOYS_ORDERS + S::first + m];
+        const int col = (S::texp && m == S::nt) ? BOYS_EXP_COL : S::first + m;
+        lds[x] = g[i * BOYS_ORDERS + col];
     }
 }
 
@@ -262,7 +266,19 @@ __device__ __forceinline__ void boys(R T, const R* tab, R (&F)[L + 1]) {
         });
     } else {
         F[L] = taylor<S::nt>(row, x);
-        const R e = dev_exp(-T);
+        R e;
+        if constexpr (S::texp && STRIDE == S::nord) {
+            // e^{-T} = e^{-T_i} e^{x} with |x| <= 0.05: nine Taylor terms of e^x (remainder
+            // below 6e-18 relative)
+            R ex = R(1) + x * R(1.0 / 8);
+            sfor<7>([&](auto K_) {
+                constexpr int k = 7 - decltype(K_)::value;  // 7 .. 1
+                ex = R(1) + (x * R(1.0 / k)) * ex;
+            });
+            e = row[S::nt] * ex;
+        } else {
+            e = dev_exp(-T);
+        }
         const R twoT = T + T;
         sfor<L>([&](auto M_) {
             constexpr int m = L - decltype(M_)::value;  // m = L .. 1
