@@ -146,6 +146,7 @@ def gradient_run(L):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eri_c20h42_mixed_traffic.json")
+CAL_COPY_DOUBLES = 1 << 26  # 512 MiB calibration copy of --profile-builds
 
 
 def measured_traffic(group):
@@ -183,6 +184,13 @@ def profile_builds(args):
     Q = eng.schwarz()
     bufs = (device.DeviceArray.from_host(2 * Da), device.DeviceArray.from_host(Da), device.DeviceArray(n * n),
             device.DeviceArray(n * n))
+    # A plain device-to-device copy of known size, so that a counter run can calibrate its
+    # FETCH_SIZE / WRITE_SIZE readings (the runtime's copy kernel, spread over the whole chip).
+    src, dst = device.DeviceArray(CAL_COPY_DOUBLES), device.DeviceArray(CAL_COPY_DOUBLES)
+    dst.copy_from(src)
+    device.synchronize()
+    src.free()
+    dst.free()
     _, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, args.cut, Q)
     roof = roofline_of(stmx, cm64, cm32)
     per_class = {CLASS_NAMES[i]: [round(cm64[i], 4), round(cm32[i], 4)] for i in range(21) if cm64[i] + cm32[i] > 0}

@@ -198,17 +198,22 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
     return x ^ (x >> 31);
 }
 
-struct ScreenArgs {
-    // bra class list
+// The bra and ket lists of one quartet class for the screening kernels (a device table of
+// N_COMBOS entries, rebuilt whenever the class lists are re-sorted).
+struct ScreenClass {
     const int* bra_ids;  // pair ids in sorted order
-    int n_bra;
-    // ket class list
+    int n_bra;           // 0: the class has no work
     const int* ket_ids;
     const double* ket_q;  // Q of each ket-list position
     const int* ket_seg_start;
     const int* ket_seg_end;
     const int* ket_molseg;  // [mol] -> first segment, [mol+1] -> end (per ket class)
-    bool diagonal;          // bra class == ket class: only ket positions <= bra position
+    int diagonal;           // bra class == ket class: only ket positions <= bra position
+    long long slot64, slot32;
+};
+
+struct ScreenArgs {
+    const ScreenClass* cls;  // [N_COMBOS]
     // per pair / shell data
     const int4* pair_info;  // {shell a, shell b, local a | local b << 16, nprim pairs}
     const int* pair_mol;
@@ -224,8 +229,7 @@ struct ScreenArgs {
     uint2* list;
     unsigned long long* counts;        // pass 0: survivors of bra i at [slot64 + i] / [slot32 + i]
     const unsigned long long* offs;    // pass 1: exclusive scan of counts = list positions
-    long long slot64, slot32;
-    unsigned long long* stats;  // [0] hash sum, [1] hash xor, [2]/[3] primitive quartets FP64/FP32
+    unsigned long long* stats;  // [4 k]: hash sum, hash xor, primitive quartets FP64 / FP32 of class k
 };
 
 // One wavefront per bra pair (a, b).  Ket candidates: for every nprim segment of the bra's
@@ -235,13 +239,24 @@ struct ScreenArgs {
 // wavefront).  PASS 0 counts each bra's survivors; after an exclusive scan of the counts,
 // PASS 1 writes them at their positions, so every class list is ordered like its bra list
 // (primitive-pair count descending: the heaviest work first).
+// All classes go out in one launch: the blocks of class k follow those of classes < k, each
+// class taking ceil(n_bra / wavefronts per block) blocks.
 template <int PASS>
 __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     extern __shared__ double srow[];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
-    const int bpos = blockIdx.x * (blockDim.x >> 6) + wib;
-    const bool active = bpos < a.n_bra;
+    const int wpb = (int)(blockDim.x >> 6);
+    int k = 0, b0 = 0;
+    for (; k < N_COMBOS; ++k) {
+        const int nb = (a.cls[k].n_bra + wpb - 1) / wpb;
+        if ((int)blockIdx.x < b0 + nb) break;
+        b0 += nb;
+    }
+    if (k == N_COMBOS) return;  // whole block beyond the last class (none launched)
+    const ScreenClass K = a.cls[k];
+    const int bpos = ((int)blockIdx.x - b0) * wpb + wib;
+    const bool active = bpos < K.n_bra;
     const int nrow = a.Mb ? 4 : 2;
     double* rows = srow + (size_t)wib * nrow * a.ns_max;
     int pid = 0, sa = 0, sb = 0, la = 0, lb = 0, mol = 0;
@@ -249,7 +264,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     double qab = 0.0, tab2 = 0.0;
     long long npa = 0;
     if (active) {
-        pid = a.bra_ids[bpos];
+        pid = K.bra_ids[bpos];
         const int4 pi = a.pair_info[pid];
         sa = pi.x;
         sb = pi.y;
@@ -276,23 +291,23 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     const double* rb = rows + a.ns_max;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0;
-    const int sg0 = a.ket_molseg[mol], sg1 = a.ket_molseg[mol + 1];
+    const int sg0 = K.ket_molseg[mol], sg1 = K.ket_molseg[mol + 1];
     unsigned long long n64 = 0, n32 = 0, b64 = 0, b32 = 0;
     if (PASS == 1) {
-        b64 = a.offs[a.slot64 + bpos];
-        b32 = a.offs[a.slot32 + bpos];
+        b64 = a.offs[K.slot64 + bpos];
+        b32 = a.offs[K.slot32 + bpos];
     }
     {
         for (int sg = sg0; sg < sg1; ++sg) {
-            const int s0 = a.ket_seg_start[sg];
-            int s1 = a.ket_seg_end[sg];
-            if (a.diagonal) s1 = min(s1, bpos + 1);
+            const int s0 = K.ket_seg_start[sg];
+            int s1 = K.ket_seg_end[sg];
+            if (K.diagonal) s1 = min(s1, bpos + 1);
             if (s1 <= s0) continue;
             // Schwarz prefix: first x in [s0, s1) with qab * Q(x) < tau_S.
             int lo = s0, hi = s1;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if (qab * a.ket_q[mid] < a.tauS)
+                if (qab * K.ket_q[mid] < a.tauS)
                     hi = mid;
                 else
                     lo = mid + 1;
@@ -305,7 +320,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                 int4 pk = make_int4(0, 0, 0, 0);
                 double mx = 0.0;
                 if (x < xend) {
-                    qid = a.ket_ids[x];
+                    qid = K.ket_ids[x];
                     pk = a.pair_info[qid];
                     const int lc = pk.z & 0xffff, ld = pk.z >> 16;
                     mx = fmax(tab2, 2.0 * a.Mt[qb + lc + (long long)ld * ns]);
@@ -316,7 +331,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                         mx = fmax(mx, fmax(fmax(rc[lc], rc[ld]), fmax(rd[lc], rd[ld])));
                     }
                     keep = !(mx < a.tauD);
-                    f32 = keep && a.mixed && (qab * a.ket_q[x] * mx < a.cut);
+                    f32 = keep && a.mixed && (qab * K.ket_q[x] * mx < a.cut);
                 }
                 const unsigned long long m64 = __ballot(keep && !f32);
                 const unsigned long long m32 = __ballot(keep && f32);
@@ -358,8 +373,8 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     }
     if (PASS == 0) {
         if (lane == 0) {
-            a.counts[a.slot64 + bpos] = n64;
-            a.counts[a.slot32 + bpos] = n32;
+            a.counts[K.slot64 + bpos] = n64;
+            a.counts[K.slot32 + bpos] = n32;
         }
         return;
     }
@@ -370,10 +385,10 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
         prims32 += __shfl_xor(prims32, off);
     }
     if (lane == 0) {
-        if (h_sum) atomicAdd(&a.stats[0], h_sum);
-        if (h_xor) atomicXor(&a.stats[1], h_xor);
-        if (prims) atomicAdd(&a.stats[2], prims);
-        if (prims32) atomicAdd(&a.stats[3], prims32);
+        if (h_sum) atomicAdd(&a.stats[4 * k + 0], h_sum);
+        if (h_xor) atomicXor(&a.stats[4 * k + 1], h_xor);
+        if (prims) atomicAdd(&a.stats[4 * k + 2], prims);
+        if (prims32) atomicAdd(&a.stats[4 * k + 3], prims32);
     }
 }
 
@@ -698,6 +713,8 @@ struct hfx_engine {
     hipGraphExec_t graph_exec = nullptr;
     BuildParams graph_key{-1.0, -1.0, -1.0, false, false, false};
     long long* d_ranges = nullptr;          // [4*N_COMBOS]
+    ScreenClass* d_screen_cls = nullptr;     // [N_COMBOS]: class lists for the screening kernels
+    int screen_nbra[N_COMBOS] = {};          // their bra counts (0: class without work)
     unsigned int* d_chunk_ctr = nullptr;     // [2*N_COMBOS*CHUNK_SLOTS]: chunk counters per class kernel
     static constexpr int CHUNK_SLOTS = 64;   // chunk kernels per class, at most
     static constexpr int CHUNK_LEN = 2048;   // list entries a block claims at a time
@@ -813,6 +830,32 @@ int sort_pairs(hfx_engine* e, const std::vector<double>& qt) {
     }
     HIPCHK(hipMemcpy(e->d_pair_q, e->pair_q.data(), e->n_pairs * sizeof(double), hipMemcpyHostToDevice));
     e->sort_q = qt;
+    // The screening kernels' class table.
+    {
+        std::vector<ScreenClass> t(N_COMBOS);
+        for (int k = 0; k < N_COMBOS; ++k) {
+            const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+            ScreenClass c{};
+            const bool live = !e->cls_ids[p1].empty() && !e->cls_ids[p2].empty();
+            c.bra_ids = e->d_cls_ids[p1];
+            c.n_bra = live ? (int)e->cls_ids[p1].size() : 0;
+            c.ket_ids = e->d_cls_ids[p2];
+            c.ket_q = e->d_cls_q[p2];
+            c.ket_seg_start = e->d_seg_start[p2];
+            c.ket_seg_end = e->d_seg_end[p2];
+            c.ket_molseg = e->d_molseg[p2];
+            c.diagonal = p1 == p2;
+            c.slot64 = e->slots[3 * k];
+            c.slot32 = e->slots[3 * k + 1];
+            t[k] = c;
+            e->screen_nbra[k] = c.n_bra;
+        }
+        if (!e->d_screen_cls) {
+            e->d_screen_cls = e->mem.alloc<ScreenClass>(N_COMBOS);
+            if (!e->d_screen_cls) return fail(HFX_ERR_NOMEM, "hipMalloc failed");
+        }
+        HIPCHK(hipMemcpy(e->d_screen_cls, t.data(), N_COMBOS * sizeof(ScreenClass), hipMemcpyHostToDevice));
+    }
     // A captured build baked in the previous order's launch shapes: capture again.
     e->graph_key.tauS = -1.0;
     return HFX_OK;
@@ -1213,51 +1256,40 @@ int enqueue_screen(hfx_engine* e, const BuildParams& P, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(std::max<long long>(1, std::min<long long>(1024, (e->n_slots + 256) / 256))),
                        dim3(256), 0, s, e->d_counts, (int)(e->n_slots + 1));
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 4 * N_COMBOS);
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1)
-            HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
-                                                    (int)(e->n_slots + 1), s));
-    for (int k = 0; k < N_COMBOS; ++k) {
-        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
-        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
-        ScreenArgs a{};
-        a.bra_ids = e->d_cls_ids[p1];
-        a.n_bra = (int)e->cls_ids[p1].size();
-        a.ket_ids = e->d_cls_ids[p2];
-        a.ket_q = e->d_cls_q[p2];
-        a.ket_seg_start = e->d_seg_start[p2];
-        a.ket_seg_end = e->d_seg_end[p2];
-        a.ket_molseg = e->d_molseg[p2];
-        a.diagonal = (p1 == p2);
-        a.pair_info = e->d_pair_info;
-        a.pair_mol = e->d_pair_mol;
-        a.pair_q = e->d_pair_q;
-        a.sh_qbase = e->d_sh_qbase;
-        a.sh_ns = e->d_sh_ns;
-        a.Mt = e->d_Mt;
-        a.Ma = e->d_Ma;
-        a.Mb = uhf ? e->d_Mb : nullptr;
-        a.ns_max = e->ns_max;
-        a.tauS = P.tauS;
-        a.tauD = P.tauD;
-        a.cut = P.cut;
-        a.mixed = mixed;
-        a.nS_total = e->nS;
-        a.list = e->d_list;
-        a.counts = e->d_counts;
-        a.offs = e->d_offs;
-        a.slot64 = e->slots[3 * k];
-        a.slot32 = e->slots[3 * k + 1];
-        a.stats = e->d_stats + 4 * k;
-        const size_t row_bytes = (size_t)(uhf ? 4 : 2) * e->ns_max * sizeof(double);
-        const int wpb = (int)std::min<size_t>(4, (60 * 1024) / row_bytes);
-        if (wpb < 1) return fail(HFX_ERR_UNSUPPORTED, "molecule too large for the screening kernel's LDS rows");
-        const dim3 sg((a.n_bra + wpb - 1) / wpb), sb(64 * wpb);
-        if (pass == 0)
-            hipLaunchKernelGGL(k_screen<0>, sg, sb, wpb * row_bytes, s, a);
-        else
-            hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
-    }
+    ScreenArgs a{};
+    a.cls = e->d_screen_cls;
+    a.pair_info = e->d_pair_info;
+    a.pair_mol = e->d_pair_mol;
+    a.pair_q = e->d_pair_q;
+    a.sh_qbase = e->d_sh_qbase;
+    a.sh_ns = e->d_sh_ns;
+    a.Mt = e->d_Mt;
+    a.Ma = e->d_Ma;
+    a.Mb = uhf ? e->d_Mb : nullptr;
+    a.ns_max = e->ns_max;
+    a.tauS = P.tauS;
+    a.tauD = P.tauD;
+    a.cut = P.cut;
+    a.mixed = mixed;
+    a.nS_total = e->nS;
+    a.list = e->d_list;
+    a.counts = e->d_counts;
+    a.offs = e->d_offs;
+    a.stats = e->d_stats;
+    const size_t row_bytes = (size_t)(uhf ? 4 : 2) * e->ns_max * sizeof(double);
+    const int wpb = (int)std::min<size_t>(4, (60 * 1024) / row_bytes);
+    if (wpb < 1) return fail(HFX_ERR_UNSUPPORTED, "molecule too large for the screening kernel's LDS rows");
+    long long blocks = 0;
+    for (int k = 0; k < N_COMBOS; ++k) blocks += (e->screen_nbra[k] + wpb - 1) / wpb;
+    if (blocks > 0) {
+        const dim3 sg((unsigned)blocks), sb(64 * wpb);
+        hipLaunchKernelGGL(k_screen<0>, sg, sb, wpb * row_bytes, s, a);
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
+                                                (int)(e->n_slots + 1), s));
+        hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
+    } else {
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
+                                                (int)(e->n_slots + 1), s));
     }
     hipLaunchKernelGGL(k_ranges, dim3(1), dim3(64), 0, s, e->d_offs, e->d_slots, e->d_ranges);
     return HFX_OK;

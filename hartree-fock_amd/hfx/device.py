@@ -26,7 +26,7 @@ def _lib():
     return _hip
 
 
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def device_count():
@@ -73,6 +73,13 @@ class DeviceArray:
         if rc != 0:
             raise RuntimeError("hipMemcpy D2H failed: %d" % rc)
         return out
+
+    def copy_from(self, other):
+        """Device-to-device copy of another buffer of the same size (the runtime's copy kernel)."""
+        assert other.n == self.n
+        rc = _lib().hipMemcpy(self.ptr, other.ptr, self.n * 8, D2D)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy D2D failed: %d" % rc)
 
     def free(self):
         if self.ptr:
