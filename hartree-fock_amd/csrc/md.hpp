@@ -171,6 +171,8 @@ __device__ __forceinline__ void hermite_r(const R (&base)[L + 1], R X, R Y, R Z,
 constexpr int BOYS_POINTS = 301;
 constexpr int BOYS_ORDERS = 18;   // columns per grid point: F_0 .. F_16, then e^{-T_i}
 constexpr int BOYS_EXP_COL = 17;
+constexpr bool BOYS_SELECT_FP32 = true;
+constexpr bool BOYS_SELECT_FP64 = false;
 
 template <class R>
 struct BoysTerms;
@@ -242,19 +244,24 @@ __device__ __forceinline__ R taylor(const R* row, R x) {
 
 // tab points at order S::first of grid point 0; consecutive grid points are STRIDE apart
 // (STRIDE = S::nord for an LDS slice, BOYS_ORDERS for the full table in global memory).
-template <int L, class R, int STRIDE = BoysSlice<L, R>::nord>
-__device__ __forceinline__ void boys(R T, const R* tab, R (&F)[L + 1]) {
+// Asymptotic form above T = 30, as the reference (src/Boys.cpp:16-23): F_0 = sqrt(pi/4T) and
+// the upward recursion without the e^{-T} term.
+template <int L, class R>
+__device__ __forceinline__ void boys_asymptotic(R T, R (&F)[L + 1]) {
+    const R rt = dev_rsqrt(T);
+    F[0] = R(0.8862269254527579) * rt;
+    const R invT = rt * rt;
+    sfor<L>([&](auto M_) {
+        constexpr int m = decltype(M_)::value;
+        F[m + 1] = (R(m) + R(0.5)) * F[m] * invT;
+    });
+}
+
+// Taylor expansion around the nearest grid point, for 0 <= T <= 30 (larger T is clamped to
+// the last point, which keeps the result finite).
+template <int L, class R, int STRIDE>
+__device__ __forceinline__ void boys_table(R T, const R* tab, R (&F)[L + 1]) {
     using S = BoysSlice<L, R>;
-    if (T > R(30)) {
-        const R rt = dev_rsqrt(T);
-        F[0] = R(0.8862269254527579) * rt;
-        const R invT = rt * rt;
-        sfor<L>([&](auto M_) {
-            constexpr int m = decltype(M_)::value;
-            F[m + 1] = (R(m) + R(0.5)) * F[m] * invT;
-        });
-        return;
-    }
     int i = (int)(T * R(10) + R(0.5));
     i = i < 0 ? 0 : (i > BOYS_POINTS - 1 ? BOYS_POINTS - 1 : i);  // NaN input stays in bounds
     const R x = R(i) * grid_step<R>() - T;
@@ -284,6 +291,36 @@ __device__ __forceinline__ void boys(R T, const R* tab, R (&F)[L + 1]) {
             constexpr int m = L - decltype(M_)::value;  // m = L .. 1
             F[m - 1] = (twoT * F[m] + e) * R(1.0 / (2 * m - 1));
         });
+    }
+}
+
+// Whether boys() evaluates both forms and selects per lane (no divergent branch) or branches.
+template <class R>
+struct BoysSelect {
+    static constexpr bool value = BOYS_SELECT_FP32;
+};
+template <>
+struct BoysSelect<double> {
+    static constexpr bool value = BOYS_SELECT_FP64;
+};
+
+template <int L, class R, int STRIDE = BoysSlice<L, R>::nord>
+__device__ __forceinline__ void boys(R T, const R* tab, R (&F)[L + 1]) {
+    if constexpr (BoysSelect<R>::value) {
+        R Fa[L + 1];
+        boys_asymptotic<L>(T, Fa);
+        boys_table<L, R, STRIDE>(T < R(30) ? T : R(30), tab, F);
+        const bool far = T > R(30);
+        sfor<L + 1>([&](auto M_) {
+            constexpr int m = decltype(M_)::value;
+            F[m] = far ? Fa[m] : F[m];
+        });
+    } else {
+        if (T > R(30)) {
+            boys_asymptotic<L>(T, F);
+            return;
+        }
+        boys_table<L, R, STRIDE>(T, tab, F);
     }
 }
 
