@@ -31,7 +31,7 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-I" + INC,
 CXXFLAGS = ["-O3", "-std=c++20", "-fPIC", "-fopenmp", "-I" + INC, "-Wall"]
 
 ACC_MAX = 36          # must match k_eri.hip
-CHUNKS_PER_TU = 3     # chunk kernels per translation unit (x2 for FP64/FP32)
+CHUNKS_PER_TU = 6     # chunks per translation unit (one launch each; x2 for FP64/FP32)
 
 
 def ncart(l):

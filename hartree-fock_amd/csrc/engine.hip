@@ -237,7 +237,8 @@ struct ScreenArgs {
 // Q(ab)Q(cd) >= tau_S holds on a prefix), then the density test per candidate.  The rows a
 // and b of the spin-density maxima are staged in LDS (dynamic shared memory, one region per
 // wavefront).  PASS 0 counts each bra's survivors; after an exclusive scan of the counts,
-// PASS 1 writes them at their positions, so every class list is ordered like its bra list
+// PASS 1 writes them at their positions (PASS 2 likewise, without the survivor-list hash of
+// the statistics), so every class list is ordered like its bra list
 // (primitive-pair count descending: the heaviest work first).
 // All classes go out in one launch: the blocks of class k follow those of classes < k, each
 // class taking ceil(n_bra / wavefronts per block) blocks.
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0;
     const int sg0 = K.ket_molseg[mol], sg1 = K.ket_molseg[mol + 1];
     unsigned long long n64 = 0, n32 = 0, b64 = 0, b32 = 0;
-    if (PASS == 1) {
+    if (PASS >= 1) {
         b64 = a.offs[K.slot64 + bpos];
         b32 = a.offs[K.slot32 + bpos];
     }
@@ -346,25 +347,27 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                         a.list[(long long)(b64 + __popcll(m64 & lt_mask))] = ent;
                     else
                         a.list[(long long)(b32 + __popcll(m32 & lt_mask))] = ent;
-                    const int sc = pk.x, sd = pk.y;
-                    // Canonical (reference loop order) form of the quartet for the hash.
-                    long long A = max(sa, sb), B = min(sa, sb), C = max(sc, sd), D = min(sc, sd);
-                    if (A * (A + 1) / 2 + B < C * (C + 1) / 2 + D) {
-                        long long t = A;
-                        A = C;
-                        C = t;
-                        t = B;
-                        B = D;
-                        D = t;
-                    }
-                    const unsigned long long h =
-                        splitmix64((unsigned long long)(((A * a.nS_total + B) * a.nS_total + C) * a.nS_total + D));
-                    h_sum += h;
-                    h_xor ^= h;
                     if (f32)
                         prims32 += (unsigned long long)(npa * pk.w);
                     else
                         prims += (unsigned long long)(npa * pk.w);
+                    if constexpr (PASS == 1) {
+                        const int sc = pk.x, sd = pk.y;
+                        // Canonical (reference loop order) form of the quartet for the hash.
+                        long long A = max(sa, sb), B = min(sa, sb), C = max(sc, sd), D = min(sc, sd);
+                        if (A * (A + 1) / 2 + B < C * (C + 1) / 2 + D) {
+                            long long t = A;
+                            A = C;
+                            C = t;
+                            t = B;
+                            B = D;
+                            D = t;
+                        }
+                        const unsigned long long h = splitmix64(
+                            (unsigned long long)(((A * a.nS_total + B) * a.nS_total + C) * a.nS_total + D));
+                        h_sum += h;
+                        h_xor ^= h;
+                    }
                 }
                 b64 += __popcll(m64);
                 b32 += __popcll(m32);
@@ -592,9 +595,10 @@ void launch_one(int pc, const OneArgs& o, hipStream_t s) {
 struct BuildParams {
     double tauS, tauD, cut;
     bool mixed, uhf, timing;
+    bool hash = true;  // the survivor-list hash (statistics only)
     bool operator==(const BuildParams& o) const {
         return tauS == o.tauS && tauD == o.tauD && cut == o.cut && mixed == o.mixed && uhf == o.uhf &&
-               timing == o.timing;
+               timing == o.timing && hash == o.hash;
     }
 };
 
@@ -1286,7 +1290,10 @@ int enqueue_screen(hfx_engine* e, const BuildParams& P, hipStream_t s) {
         hipLaunchKernelGGL(k_screen<0>, sg, sb, wpb * row_bytes, s, a);
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
                                                 (int)(e->n_slots + 1), s));
-        hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
+        if (P.hash)
+            hipLaunchKernelGGL(k_screen<1>, sg, sb, wpb * row_bytes, s, a);
+        else
+            hipLaunchKernelGGL(k_screen<2>, sg, sb, wpb * row_bytes, s, a);
     } else {
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(e->d_scan_tmp, e->scan_tmp_bytes, e->d_counts, e->d_offs,
                                                 (int)(e->n_slots + 1), s));
@@ -1483,6 +1490,7 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     BuildParams P{args->schwarz_thresh, args->density_thresh, args->fp32_cut,
                   args->precision == HFX_MIXED && args->fp32_cut > 0.0, uhf,
                   (args->flags & HFX_JK_CLASS_TIMING) != 0};
+    P.hash = st != nullptr && !(args->flags & HFX_JK_NO_STATS);
     int ev_slot[2 * N_COMBOS];
     int n_ev = 0;
     const bool timing = P.timing;
