@@ -1363,12 +1363,10 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     hipLaunchKernelGGL(k_fill_u32, dim3(4), dim3(256), 0, s, e->d_chunk_ctr, 2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
     ea.chunk_len = hfx_engine::CHUNK_LEN;
     // LDS exchange rows of a class kernel: NA + NB rows of max_N doubles (twice for UHF), next
-    // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without,
-    // and so does (ss|ss), whose four exchange terms per quartet cost less than the rows.
+    // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without.
     auto tile_cols = [&](int k, int fp32) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
-        if (L == 0) return 0;
         const int nt = fp32 ? 5 : 8, nord = L <= 1 ? L + nt : nt + (fp32 ? 0 : 1);
         const size_t boys = (size_t)md::BOYS_POINTS * nord * (fp32 ? 4 : 8);
         const size_t rows = (size_t)(uhf ? 2 : 1) * (ncart_h(la) + ncart_h(lb)) * e->max_N * sizeof(double);
