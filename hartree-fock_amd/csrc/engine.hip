@@ -1116,6 +1116,10 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     }
     e->n_pp = (long long)pp64.size();
     e->n_pp32 = (long long)pp32.size();
+    // One zero element past the end: the class kernels load the primitive pair after the one
+    // they use one iteration ahead, without a bounds clamp (its kp is 0).
+    pp64.push_back(PrimPair64{});
+    pp32.push_back(PrimPair32{});
     std::vector<int4> pinfo(e->n_pairs);
     for (int p = 0; p < e->n_pairs; ++p) {
         const int a = e->pair_a[p], b = e->pair_b[p];
@@ -1367,8 +1371,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     auto tile_cols = [&](int k, int fp32) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
-        const int nt = fp32 ? 5 : 8, nord = L <= 1 ? L + nt : nt + (fp32 ? 0 : 1);
-        const size_t boys = (size_t)md::BOYS_POINTS * nord * (fp32 ? 4 : 8);
+        const size_t boys = (size_t)md::BOYS_POINTS * md::boys_slice_nord(L, !fp32) * (fp32 ? 4 : 8);
         const size_t rows = (size_t)(uhf ? 2 : 1) * (ncart_h(la) + ncart_h(lb)) * e->max_N * sizeof(double);
         const size_t mask = (size_t)((e->ns_max + 31) / 32) * sizeof(unsigned int);
         return boys + rows + mask + 1024 <= 64 * 1024 ? e->max_N : 0;

@@ -185,6 +185,10 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
     // last one gets a pass of its own.
     int iq = 0;
     if constexpr (KET_PAIRS) {
+        // In FP32 the bra loop is unrolled by two with the loads alternating between A0 and A1,
+        // so no register copies are needed for the one-ahead prefetch; pp[ps + np] is the next
+        // pair's first primitive pair or the zero element past the end: always readable.
+        const PrimPair<R>* __restrict__ pa = pp + ps;
         for (; iq + 1 < nq; iq += 2) {
             const PrimPair<R> B0 = pp[qs + iq], B1 = pp[qs + iq + 1];
             Ecoef<LC, LD, R> Fx0, Fy0, Fz0, Fx1, Fy1, Fz1;
@@ -194,17 +198,33 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             hermite_e<LC, LD>(B1.oo2p, B1.pax, B1.pbx, Fx1);
             hermite_e<LC, LD>(B1.oo2p, B1.pay, B1.pby, Fy1);
             hermite_e<LC, LD>(B1.oo2p, B1.paz, B1.pbz, Fz1);
-            PrimPair<R> A;
-            if (np > 0) A = pp[ps];
-            for (int ip = 0; ip < np; ++ip) {
-                const PrimPair<R> An = pp[ps + (ip + 1 < np ? ip + 1 : ip)];
+            auto both = [&](const PrimPair<R>& A) {
                 Ecoef<LA, LB, R> Ex, Ey, Ez;
                 hermite_e<LA, LB>(A.oo2p, A.pax, A.pbx, Ex);
                 hermite_e<LA, LB>(A.oo2p, A.pay, A.pby, Ey);
                 hermite_e<LA, LB>(A.oo2p, A.paz, A.pbz, Ez);
                 prim(A, Ex, Ey, Ez, B0, Fx0, Fy0, Fz0);
                 prim(A, Ex, Ey, Ez, B1, Fx1, Fy1, Fz1);
-                A = An;
+            };
+            if constexpr (sizeof(R) == 4) {
+                PrimPair<R> A0 = pa[0];
+                int ip = 0;
+                for (; ip + 1 < np; ip += 2) {
+                    const PrimPair<R> A1 = pa[ip + 1];
+                    both(A0);
+                    A0 = pa[ip + 2];
+                    both(A1);
+                }
+                if (ip < np) both(A0);
+            } else {
+                // FP64: the register cost of the unrolled loop outweighs its savings
+                PrimPair<R> A;
+                if (np > 0) A = pa[0];
+                for (int ip = 0; ip < np; ++ip) {
+                    const PrimPair<R> An = pa[ip + 1 < np ? ip + 1 : ip];
+                    both(A);
+                    A = An;
+                }
             }
         }
     }

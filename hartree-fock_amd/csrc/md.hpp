@@ -185,13 +185,26 @@ struct BoysTerms<float> {
     static constexpr int nt = 5;  // remainder < 3e-9 relative
 };
 
+// Layout of the LDS slice of a kernel whose highest order is L.  An FP64 slice used with the
+// downward recursion also carries e^{-T_i} (its last column).  Taylor coefficients are stored
+// divided by k! (one FMA per term), one run of nt per order evaluated from the table; the FP64
+// slice of L = 1 keeps the orders shared (F_0..F_8 once), which halves its LDS.
+__host__ __device__ constexpr bool boys_slice_texp(int L, bool fp64) { return L >= 2 && fp64; }
+__host__ __device__ constexpr bool boys_slice_pre(int L, bool fp64) { return !(L == 1 && fp64); }
+__host__ __device__ constexpr int boys_slice_nord(int L, bool fp64) {
+    const int nt = fp64 ? 8 : 5;
+    const int ntab = L <= 1 ? L + 1 : 1;  // orders evaluated from the table
+    return (boys_slice_pre(L, fp64) ? ntab * nt : (L <= 1 ? L + nt : nt)) + (boys_slice_texp(L, fp64) ? 1 : 0);
+}
+
 template <int L, class R>
 struct BoysSlice {
     static constexpr int nt = BoysTerms<R>::nt;
     static constexpr int first = (L <= 1) ? 0 : L;
-    // An FP64 slice used with the downward recursion also carries e^{-T_i} (its last column).
-    static constexpr bool texp = L >= 2 && sizeof(R) == 8;
-    static constexpr int nord = (L <= 1) ? L + nt : nt + (texp ? 1 : 0);
+    static constexpr bool texp = boys_slice_texp(L, sizeof(R) == 8);
+    static constexpr bool pre = boys_slice_pre(L, sizeof(R) == 8);
+    static constexpr int nord = boys_slice_nord(L, sizeof(R) == 8);
+    static_assert(nt == (sizeof(R) == 8 ? 8 : 5), "boys_slice_nord assumes these term counts");
 };
 
 template <class R>
@@ -224,11 +237,32 @@ __device__ __forceinline__ float dev_exp<float>(float x) { return __expf(x); }
 template <int L, class R>
 __device__ __forceinline__ void load_boys(const R* __restrict__ g, R* lds) {
     using S = BoysSlice<L, R>;
+    constexpr R inv_fact[10] = {R(1.0), R(1.0), R(1.0 / 2), R(1.0 / 6), R(1.0 / 24), R(1.0 / 120), R(1.0 / 720),
+                                R(1.0 / 5040), R(1.0 / 40320), R(1.0 / 362880)};
     for (int x = threadIdx.x; x < BOYS_POINTS * S::nord; x += blockDim.x) {
         const int i = x / S::nord, m = x - i * S::nord;
-        const int col = (S::texp && m == S::nt) ? BOYS_EXP_COL : S::first + m;
-        lds[x] = g[i * BOYS_ORDERS + col];
+        R v;
+        if (S::texp && m == S::nord - 1) {
+            v = g[i * BOYS_ORDERS + BOYS_EXP_COL];
+        } else if (S::pre) {
+            const int o = m / S::nt, k = m - o * S::nt;  // order first + o, Taylor term k
+            v = g[i * BOYS_ORDERS + S::first + o + k] * inv_fact[k];
+        } else {
+            v = g[i * BOYS_ORDERS + S::first + m];
+        }
+        lds[x] = v;
     }
+}
+
+// Sum of row[k] x^k, k < NT (coefficients already divided by k!).
+template <int NT, class R>
+__device__ __forceinline__ R horner(const R* row, R x) {
+    R t = row[NT - 1];
+    sfor<NT - 1>([&](auto K_) {
+        constexpr int k = NT - 2 - decltype(K_)::value;
+        t = row[k] + x * t;
+    });
+    return t;
 }
 
 // Taylor sum of row[0..nt) at offset x = T0 - T: sum_k row[k] x^k / k!.
@@ -266,23 +300,28 @@ __device__ __forceinline__ void boys_table(R T, const R* tab, R (&F)[L + 1]) {
     i = i < 0 ? 0 : (i > BOYS_POINTS - 1 ? BOYS_POINTS - 1 : i);  // NaN input stays in bounds
     const R x = R(i) * grid_step<R>() - T;
     const R* row = tab + i * STRIDE;
+    constexpr bool slice = STRIDE == S::nord;  // LDS slice (else the full table in global memory)
     if constexpr (L <= 1) {
         sfor<L + 1>([&](auto M_) {
             constexpr int m = decltype(M_)::value;
-            F[m] = taylor<S::nt>(row + m, x);
+            if constexpr (slice && S::pre)
+                F[m] = horner<S::nt>(row + m * S::nt, x);
+            else
+                F[m] = taylor<S::nt>(row + m, x);
         });
     } else {
-        F[L] = taylor<S::nt>(row, x);
+        if constexpr (slice && S::pre)
+            F[L] = horner<S::nt>(row, x);
+        else
+            F[L] = taylor<S::nt>(row, x);
         R e;
-        if constexpr (S::texp && STRIDE == S::nord) {
+        if constexpr (S::texp && slice) {
             // e^{-T} = e^{-T_i} e^{x} with |x| <= 0.05: nine Taylor terms of e^x (remainder
             // below 6e-18 relative)
-            R ex = R(1) + x * R(1.0 / 8);
-            sfor<7>([&](auto K_) {
-                constexpr int k = 7 - decltype(K_)::value;  // 7 .. 1
-                ex = R(1) + (x * R(1.0 / k)) * ex;
-            });
-            e = row[S::nt] * ex;
+            constexpr R c[9] = {R(1.0), R(1.0), R(1.0 / 2), R(1.0 / 6), R(1.0 / 24), R(1.0 / 120), R(1.0 / 720),
+                                R(1.0 / 5040), R(1.0 / 40320)};
+            R ex = horner<9>(c, x);
+            e = row[S::nord - 1] * ex;
         } else {
             e = dev_exp(-T);
         }
