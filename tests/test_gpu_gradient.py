@@ -46,7 +46,9 @@ def test_water8_gradient_against_finite_differences(engine_lib):
     r, g, parts, st = gt.analytic(engine_lib, b, **kw)
     fd = gt.finite_differences_batched(engine_lib, b, **kw)
     err = np.abs(fd - g).max()
-    print("max |analytic - FD| = %.2e Eh/Bohr, max |g| = %.2e" % (err, np.abs(g).max()), flush=True)
+    A, x = np.unravel_index(np.argmax(np.abs(fd - g)), g.shape)
+    print("max |analytic - FD| = %.2e Eh/Bohr at atom %d, axis %d; max |g| = %.2e" % (err, A, x, np.abs(g).max()),
+          flush=True)
     assert err <= 1e-6
     assert np.abs(g.sum(0)).max() <= 1e-9
 
