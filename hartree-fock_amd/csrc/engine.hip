@@ -119,7 +119,7 @@ namespace {
 constexpr int CMAX = 6;  // Cartesian components of a d shell
 
 // ---------------------------------------------------------------------------------------
-// Boys table: F_m(i * 0.1), i = 0..300, m = 0..16, and e^{-i * 0.1} in the last column, from the series
+// Boys tables: F_m(T_i), m = 0..16, and e^{-T_i} in the last column, from the series
 // F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double.
 // ---------------------------------------------------------------------------------------
 long double boys_series_ld(int m, long double T) {
@@ -1128,16 +1128,20 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     for (int m = 0; m < n_mol; ++m) e->ns_max = std::max(e->ns_max, e->mol_nS[m]);
     if (e->ns_max > 65535) return fail(HFX_ERR_UNSUPPORTED, "more than 65535 shells in one molecule");
     // ---- Boys tables ----
-    std::vector<double> b64((size_t)md::BOYS_POINTS * md::BOYS_ORDERS);
-    std::vector<float> b32((size_t)md::BOYS_POINTS * md::BOYS_ORDERS);
-    for (int i = 0; i < md::BOYS_POINTS; ++i) {
-        const double T64 = i * 0.1;
-        const float T32 = (float)i * 0.1f;
-        for (int m = 0; m < md::BOYS_EXP_COL; ++m) {
+    // FP64 at T_i = 0.1 i (i <= 300), FP32 at T_i = 0.05 i (i <= 600), each grid point as the
+    // kernels form it in their precision.
+    std::vector<double> b64((size_t)md::BoysGrid<double>::points * md::BOYS_ORDERS);
+    std::vector<float> b32((size_t)md::BoysGrid<float>::points * md::BOYS_ORDERS);
+    for (int i = 0; i < md::BoysGrid<double>::points; ++i) {
+        const double T64 = (double)i * md::BoysGrid<double>::step;
+        for (int m = 0; m < md::BOYS_EXP_COL; ++m)
             b64[(size_t)i * md::BOYS_ORDERS + m] = (double)boys_series_ld(m, (long double)T64);
-            b32[(size_t)i * md::BOYS_ORDERS + m] = (float)boys_series_ld(m, (long double)T32);
-        }
         b64[(size_t)i * md::BOYS_ORDERS + md::BOYS_EXP_COL] = (double)std::exp(-(long double)T64);
+    }
+    for (int i = 0; i < md::BoysGrid<float>::points; ++i) {
+        const float T32 = (float)i * md::BoysGrid<float>::step;
+        for (int m = 0; m < md::BOYS_EXP_COL; ++m)
+            b32[(size_t)i * md::BOYS_ORDERS + m] = (float)boys_series_ld(m, (long double)T32);
         b32[(size_t)i * md::BOYS_ORDERS + md::BOYS_EXP_COL] = (float)std::exp(-(long double)T32);
     }
 
@@ -1371,7 +1375,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     auto tile_cols = [&](int k, int fp32) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
-        const size_t boys = (size_t)md::BOYS_POINTS * md::boys_slice_nord(L, !fp32) * (fp32 ? 4 : 8);
+        const size_t boys = (size_t)md::boys_points(!fp32) * md::boys_slice_nord(L, !fp32) * (fp32 ? 4 : 8);
         const size_t rows = (size_t)(uhf ? 2 : 1) * (ncart_h(la) + ncart_h(lb)) * e->max_N * sizeof(double);
         const size_t mask = (size_t)((e->ns_max + 31) / 32) * sizeof(unsigned int);
         return boys + rows + mask + 1024 <= 64 * 1024 ? e->max_N : 0;

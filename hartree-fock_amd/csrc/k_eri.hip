@@ -526,7 +526,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
 template <int KC, class R, int CH0, int NC>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     using S = BoysSlice<LTOT, R>;
-    __shared__ R tab[BOYS_POINTS * S::nord];
+    __shared__ __align__(16) R tab[BoysGrid<R>::points * S::nord];
     __shared__ long long s_chunk;
     extern __shared__ double tile_mem[];
     const int ch = (int)(blockIdx.x % NC), vb = (int)(blockIdx.x / NC), nvb = (int)(gridDim.x / NC);

@@ -60,7 +60,7 @@ __device__ void build_e(R* E, int li, int lj, R pa, R pb, R oo2p) {
 }
 
 // F_n(T) for one order n per lane: asymptotic above T = 30, else a Taylor expansion around
-// the nearest point of the 0.1-spaced table (global memory, BOYS_ORDERS per point).
+// the nearest point of the precision's table (global memory, BOYS_ORDERS per point).
 template <class R>
 __device__ R boys_one(int n, R T, const R* __restrict__ tab) {
     if (T > R(30)) {
@@ -71,9 +71,10 @@ __device__ R boys_one(int n, R T, const R* __restrict__ tab) {
         return f;
     }
     constexpr int NT = md::BoysTerms<R>::nt;
-    int i = (int)(T * R(10) + R(0.5));
-    i = i < 0 ? 0 : (i > md::BOYS_POINTS - 1 ? md::BOYS_POINTS - 1 : i);
-    const R x = R(i) * md::grid_step<R>() - T;
+    using G = md::BoysGrid<R>;
+    int i = (int)(T * G::inv_step + R(0.5));
+    i = i < 0 ? 0 : (i > G::points - 1 ? G::points - 1 : i);
+    const R x = R(i) * G::step - T;
     const R* row = tab + i * md::BOYS_ORDERS + n;
     R t = row[NT - 1];
     for (int k = NT - 2; k >= 0; --k) t = row[k] + (x * (R(1) / R(k + 1))) * t;

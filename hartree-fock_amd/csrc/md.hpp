@@ -165,10 +165,11 @@ __device__ __forceinline__ void hermite_r(const R (&base)[L + 1], R X, R Y, R Z,
 }
 
 // ---------------------------------------------------------------------------------------
-// Boys function.  The table holds F_m(i * 0.1) for i = 0..300, m = 0..BOYS_ORDERS-1.
-// A kernel whose highest order is L copies the orders [first, first+nord) into LDS.
+// Boys function.  The tables hold F_m(i h), m = 0..16, and e^{-i h} at the grid points of
+// BoysGrid (h = 0.1 up to T = 30 in FP64, h = 0.05 in FP32).  A kernel whose highest order is
+// L copies the Taylor coefficients of the orders it evaluates into LDS (BoysSlice).
 // ---------------------------------------------------------------------------------------
-constexpr int BOYS_POINTS = 301;
+constexpr int BOYS_POINTS = 301;  // FP64 grid
 constexpr int BOYS_ORDERS = 18;   // columns per grid point: F_0 .. F_16, then e^{-T_i}
 constexpr int BOYS_EXP_COL = 17;
 constexpr bool BOYS_SELECT_FP32 = true;
@@ -182,8 +183,22 @@ struct BoysTerms<double> {
 };
 template <>
 struct BoysTerms<float> {
-    static constexpr int nt = 5;  // remainder < 3e-9 relative
+    static constexpr int nt = 4;  // |dT| <= 0.025: remainder < 2e-8 relative
 };
+
+template <class R>
+struct BoysGrid;
+template <>
+struct BoysGrid<double> {
+    static constexpr int points = BOYS_POINTS;
+    static constexpr double step = 0.1, inv_step = 10.0;
+};
+template <>
+struct BoysGrid<float> {
+    static constexpr int points = 601;
+    static constexpr float step = 0.05f, inv_step = 20.0f;
+};
+__host__ __device__ constexpr int boys_points(bool fp64) { return fp64 ? 301 : 601; }
 
 // Layout of the LDS slice of a kernel whose highest order is L.  An FP64 slice used with the
 // downward recursion also carries e^{-T_i} (its last column).  Taylor coefficients are stored
@@ -192,7 +207,7 @@ struct BoysTerms<float> {
 __host__ __device__ constexpr bool boys_slice_texp(int L, bool fp64) { return L >= 2 && fp64; }
 __host__ __device__ constexpr bool boys_slice_pre(int L, bool fp64) { return !(L == 1 && fp64); }
 __host__ __device__ constexpr int boys_slice_nord(int L, bool fp64) {
-    const int nt = fp64 ? 8 : 5;
+    const int nt = fp64 ? 8 : 4;
     const int ntab = L <= 1 ? L + 1 : 1;  // orders evaluated from the table
     return (boys_slice_pre(L, fp64) ? ntab * nt : (L <= 1 ? L + nt : nt)) + (boys_slice_texp(L, fp64) ? 1 : 0);
 }
@@ -204,15 +219,10 @@ struct BoysSlice {
     static constexpr bool texp = boys_slice_texp(L, sizeof(R) == 8);
     static constexpr bool pre = boys_slice_pre(L, sizeof(R) == 8);
     static constexpr int nord = boys_slice_nord(L, sizeof(R) == 8);
-    static_assert(nt == (sizeof(R) == 8 ? 8 : 5), "boys_slice_nord assumes these term counts");
+    static_assert(nt == (sizeof(R) == 8 ? 8 : 4), "boys_slice_nord assumes these term counts");
+    static_assert(BoysGrid<R>::points == boys_points(sizeof(R) == 8), "grid sizes");
 };
 
-template <class R>
-__device__ __forceinline__ R grid_step();
-template <>
-__device__ __forceinline__ double grid_step<double>() { return 0.1; }
-template <>
-__device__ __forceinline__ float grid_step<float>() { return 0.1f; }
 
 // 1/sqrt(x): the hardware estimate, refined by two Newton steps in FP64 (relative error of
 // a few ulp); the single-precision estimate is used as is in FP32.
@@ -239,7 +249,7 @@ __device__ __forceinline__ void load_boys(const R* __restrict__ g, R* lds) {
     using S = BoysSlice<L, R>;
     constexpr R inv_fact[10] = {R(1.0), R(1.0), R(1.0 / 2), R(1.0 / 6), R(1.0 / 24), R(1.0 / 120), R(1.0 / 720),
                                 R(1.0 / 5040), R(1.0 / 40320), R(1.0 / 362880)};
-    for (int x = threadIdx.x; x < BOYS_POINTS * S::nord; x += blockDim.x) {
+    for (int x = threadIdx.x; x < BoysGrid<R>::points * S::nord; x += blockDim.x) {
         const int i = x / S::nord, m = x - i * S::nord;
         R v;
         if (S::texp && m == S::nord - 1) {
@@ -296,9 +306,10 @@ __device__ __forceinline__ void boys_asymptotic(R T, R (&F)[L + 1]) {
 template <int L, class R, int STRIDE>
 __device__ __forceinline__ void boys_table(R T, const R* tab, R (&F)[L + 1]) {
     using S = BoysSlice<L, R>;
-    int i = (int)(T * R(10) + R(0.5));
-    i = i < 0 ? 0 : (i > BOYS_POINTS - 1 ? BOYS_POINTS - 1 : i);  // NaN input stays in bounds
-    const R x = R(i) * grid_step<R>() - T;
+    using G = BoysGrid<R>;
+    int i = (int)(T * G::inv_step + R(0.5));
+    i = i < 0 ? 0 : (i > G::points - 1 ? G::points - 1 : i);  // NaN input stays in bounds
+    const R x = R(i) * G::step - T;
     const R* row = tab + i * STRIDE;
     constexpr bool slice = STRIDE == S::nord;  // LDS slice (else the full table in global memory)
     if constexpr (L <= 1) {
