@@ -891,7 +891,15 @@ EriArgs eri_args(hfx_engine* e) {
     return a;
 }
 
-int eri_grid(hfx_engine* e) { return e->n_cu * 8; }
+// Blocks per class launch: two per CU (HFX_BLOCKS_PER_CU to change).  The launches of a build
+// run four at a time on the streams, and small persistent grids let them share the machine.
+int eri_grid(hfx_engine* e) {
+    static const int mult = [] {
+        const char* v = std::getenv("HFX_BLOCKS_PER_CU");
+        return v ? std::max(1, std::min(64, std::atoi(v))) : 2;
+    }();
+    return e->n_cu * mult;
+}
 
 // Store-mode ERI blocks for a list of (bra pair, ket pair) entries of one class.
 int store_blocks(hfx_engine* e, int combo, const std::vector<uint2>& ents, std::vector<double>& out,

@@ -484,8 +484,9 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
         for (int w = threadIdx.x; w < (a.tile_shells + 31) / 32; w += blockDim.x)
             ((unsigned int*)(tile_mem + n_tiles * (NA + NB) * a.tile_n))[w] = 0u;
     int tile_pid = -1;
-    // Chunks of at most chunk_len entries, shorter for short lists so every block gets work.
-    long long L = (n + nvb - 1) / nvb;
+    // Chunks of at most chunk_len entries, shorter for short lists so that every block claims
+    // about four (the lists are sorted heaviest first: several claims per block balance them).
+    long long L = (n + 4LL * nvb - 1) / (4LL * nvb);
     L = ((L + blockDim.x - 1) / blockDim.x) * blockDim.x;
     if (L > a.chunk_len) L = a.chunk_len;
     for (;;) {
