@@ -61,11 +61,11 @@ int fail(int code, const std::string& msg) {
 // Registry of ERI class kernels (filled by the k_eri translation units at load time).
 // ---------------------------------------------------------------------------------------
 struct PartTable {
-    EriPartFn fn[N_COMBOS][64];
-    int begin[N_COMBOS][64];
-    int end[N_COMBOS][64];
-    int nparts[N_COMBOS];
-    int nch[N_COMBOS];
+    EriPartFn fn[2][N_COMBOS][64];
+    int begin[2][N_COMBOS][64];
+    int end[2][N_COMBOS][64];
+    int nparts[2][N_COMBOS];
+    int nch[2][N_COMBOS];
 };
 PartTable& parts() {
     static PartTable t;  // zero-initialized before any dynamic initializer runs
@@ -76,40 +76,40 @@ PartTable& parts() {
 
 namespace hfx {
 
-bool register_eri_part(int combo, int begin, int end, int nch, EriPartFn fn) {
+bool register_eri_part(int combo, int fp32, int begin, int end, int nch, EriPartFn fn) {
     PartTable& t = parts();
-    if (combo < 0 || combo >= N_COMBOS || t.nparts[combo] >= 64) return false;
-    const int k = t.nparts[combo]++;
-    t.fn[combo][k] = fn;
-    t.begin[combo][k] = begin;
-    t.end[combo][k] = end;
-    t.nch[combo] = nch;
+    if (combo < 0 || combo >= N_COMBOS || fp32 < 0 || fp32 > 1 || t.nparts[fp32][combo] >= 64) return false;
+    const int k = t.nparts[fp32][combo]++;
+    t.fn[fp32][combo][k] = fn;
+    t.begin[fp32][combo][k] = begin;
+    t.end[fp32][combo][k] = end;
+    t.nch[fp32][combo] = nch;
     return true;
 }
 
-int eri_chunks(int combo) {
+int eri_chunks(int combo, int fp32) {
     PartTable& t = parts();
-    const int n = t.nch[combo];
+    const int n = t.nch[fp32][combo];
     std::vector<int> cover(n > 0 ? n : 0, 0);
-    for (int k = 0; k < t.nparts[combo]; ++k)
-        for (int c = t.begin[combo][k]; c < t.end[combo][k] && c < n; ++c) cover[c]++;
+    for (int k = 0; k < t.nparts[fp32][combo]; ++k)
+        for (int c = t.begin[fp32][combo][k]; c < t.end[fp32][combo][k] && c < n; ++c) cover[c]++;
     for (int c = 0; c < n; ++c)
         if (cover[c] != 1) return -1;
     return n > 0 ? n : -1;
 }
 
-int eri_parts(int combo, int* begin, int* end) {
+int eri_parts(int combo, int fp32, int* begin, int* end) {
     PartTable& t = parts();
-    for (int k = 0; k < t.nparts[combo]; ++k) {
-        begin[k] = t.begin[combo][k];
-        end[k] = std::min(t.end[combo][k], t.nch[combo]);
+    for (int k = 0; k < t.nparts[fp32][combo]; ++k) {
+        begin[k] = t.begin[fp32][combo][k];
+        end[k] = std::min(t.end[fp32][combo][k], t.nch[fp32][combo]);
     }
-    return t.nparts[combo];
+    return t.nparts[fp32][combo];
 }
 
 void launch_eri(int combo, int fp32, const EriArgs& a, int grid, const hipStream_t* chunk_streams) {
     PartTable& t = parts();
-    for (int k = 0; k < t.nparts[combo]; ++k) t.fn[combo][k](fp32, a, grid, chunk_streams);
+    for (int k = 0; k < t.nparts[fp32][combo]; ++k) t.fn[fp32][combo][k](fp32, a, grid, chunk_streams);
 }
 
 }  // namespace hfx
@@ -911,7 +911,7 @@ int store_blocks(hfx_engine* e, int combo, const std::vector<uint2>& ents, std::
     for (long long i = 0; i < n; ++i) offs[i] = i * blk;
     out.assign(n * blk, 0.0);
     if (n == 0) return HFX_OK;
-    if (eri_chunks(combo) < 0) return fail(HFX_ERR_UNSUPPORTED, "ERI kernels missing for class " + std::to_string(combo));
+    if (eri_chunks(combo, 0) < 0) return fail(HFX_ERR_UNSUPPORTED, "ERI kernels missing for class " + std::to_string(combo));
     DevArena tmp;
     uint2* dl = tmp.alloc<uint2>(n);
     long long* doff = tmp.alloc<long long>(n);
@@ -927,7 +927,7 @@ int store_blocks(hfx_engine* e, int combo, const std::vector<uint2>& ents, std::
     a.range = drange;
     a.out = dout;
     a.out_off = doff;
-    std::vector<hipStream_t> cs(std::max(1, eri_chunks(combo)), e->stream);
+    std::vector<hipStream_t> cs(std::max(1, eri_chunks(combo, 0)), e->stream);
     launch_eri(combo, 0, a, eri_grid(e), cs.data());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
@@ -973,7 +973,8 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     if (!out || !bases || n_mol <= 0) return fail(HFX_ERR_INVALID, "hfx_create: bad arguments");
     *out = nullptr;
     for (int c = 0; c < N_COMBOS; ++c)
-        if (eri_chunks(c) < 0) return fail(HFX_ERR_UNSUPPORTED, "library built without the kernels of class " + std::to_string(c));
+        if (eri_chunks(c, 0) < 0 || eri_chunks(c, 1) < 0)
+            return fail(HFX_ERR_UNSUPPORTED, "library built without the kernels of class " + std::to_string(c));
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(HFX_ERR_INVALID, "hfx_create: no such device");
@@ -1338,7 +1339,7 @@ double chunk_cost(hfx_engine* e, int k, int fp32, bool mixed) {
     const double per = mean_np(b) * mean_np(c) * model_flop_h(la, lb, lc, ld) +
                        12.0 * ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
     const double share = !mixed ? 1.0 : (fp32 ? 0.9 : 0.25);
-    const int nch = std::max(1, eri_chunks(k));
+    const int nch = std::max(1, eri_chunks(k, fp32));
     return 1e-9 * quartets * per * share / nch + 1.0;
 }
 
@@ -1401,9 +1402,9 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         std::vector<Item> items;
         int pb[64], pe[64];
         for (int k = 0; k < N_COMBOS; ++k) {
-            const int nch = eri_chunks(k);
-            const int np = eri_parts(k, pb, pe);
             for (int f = 0; f < (mixed ? 2 : 1); ++f) {
+                const int nch = eri_chunks(k, f);
+                const int np = eri_parts(k, f, pb, pe);
                 cs[2 * k + f].assign(std::max(1, nch), s);
                 const double c = chunk_cost(e, k, f, mixed);
                 for (int x = 0; x < np; ++x)

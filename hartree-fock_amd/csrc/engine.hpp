@@ -140,12 +140,13 @@ void launch_grad_reduce(const GradArgs& g, int n_rows, double* out, hipStream_t 
 // Each kernel translation unit compiles chunks [begin, end) of one class and registers a
 // launcher for them at load time; launch_eri runs every registered part of a class, each as
 // one launch (on the stream of its first chunk, chunk_streams[begin]).
+// A class has nch chunks in each precision (fewer in FP32 for the heavy classes).
 using EriPartFn = void (*)(int fp32, const EriArgs& a, int grid, const hipStream_t* chunk_streams);
-bool register_eri_part(int combo, int begin, int end, int nch, EriPartFn fn);
-// Number of chunk kernels of a class, or -1 when its parts do not cover [0, nch).
-int eri_chunks(int combo);
-// Chunk ranges [begin[k], end[k]) of the class's launches; returns their number (<= 64).
-int eri_parts(int combo, int* begin, int* end);
+bool register_eri_part(int combo, int fp32, int begin, int end, int nch, EriPartFn fn);
+// Number of chunks of a class in a precision, or -1 when its parts do not cover [0, nch).
+int eri_chunks(int combo, int fp32);
+// Chunk ranges [begin[k], end[k]) of the class's launches in a precision; returns their number (<= 64).
+int eri_parts(int combo, int fp32, int* begin, int* end);
 void launch_eri(int combo, int fp32, const EriArgs& a, int grid, const hipStream_t* chunk_streams);
 
 }  // namespace hfx

@@ -48,10 +48,18 @@ constexpr int PC1 = combo_pc1(K_COMBO), PC2 = combo_pc2(K_COMBO);
 constexpr int LA = pc_la(PC1), LB = pc_lb(PC1), LC = pc_la(PC2), LD = pc_lb(PC2);
 constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
 constexpr int NAB = NA * NB, NCD = NC * ND;
-constexpr int ACC_MAX = 36;
-constexpr int CB_MAX = cmax(1, ACC_MAX / NCD);
-constexpr int NCH = (NAB + CB_MAX - 1) / CB_MAX;
-constexpr int CB = (NAB + NCH - 1) / NCH;
+// Chunking of the bra components: a thread accumulates at most `acc` integrals (cb bra
+// components x all NCD ket components), so a class is split into nch chunks.  FP32 threads of
+// the classes with L >= 4 hold twice as many (they run at one wavefront per SIMD either way,
+// and fewer chunks repeat the Boys function, R and the E tables fewer times).
+template <class R>
+struct Chunks {
+    static constexpr int acc = (sizeof(R) == 4 && LA + LB + LC + LD >= 4) ? 72 : 36;
+    static constexpr int cb_max = cmax(1, acc / NCD);
+    static constexpr int nch = (NAB + cb_max - 1) / cb_max;
+    static constexpr int cb = (NAB + nch - 1) / nch;
+};
+constexpr int NCH = Chunks<double>::nch;  // the translation units are sliced by FP64 chunks
 constexpr int LTOT = LA + LB + LC + LD;
 constexpr int LCD = LC + LD;
 constexpr double TWO_PI_2_5 = 34.98683665524972;  // 2 pi^(5/2)
@@ -84,13 +92,13 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
 __device__ __forceinline__ int xcd_id() { return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u); }
 
 // Is bra component row ia (of shell a) / ib (of shell b) touched by chunk C0?
-__host__ __device__ constexpr bool chunk_has_a(int c0, int ia) {
-    for (int ab = c0; ab < c0 + CB && ab < NAB; ++ab)
+__host__ __device__ constexpr bool chunk_has_a(int c0, int cb, int ia) {
+    for (int ab = c0; ab < c0 + cb && ab < NAB; ++ab)
         if (ab / NB == ia) return true;
     return false;
 }
-__host__ __device__ constexpr bool chunk_has_b(int c0, int ib) {
-    for (int ab = c0; ab < c0 + CB && ab < NAB; ++ab)
+__host__ __device__ constexpr bool chunk_has_b(int c0, int cb, int ib) {
+    for (int ab = c0; ab < c0 + cb && ab < NAB; ++ab)
         if (ab % NB == ib) return true;
     return false;
 }
@@ -100,6 +108,7 @@ __host__ __device__ constexpr bool chunk_has_b(int c0, int ib) {
 template <class R, int C0>
 __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab, double* tile,
                                         int tile_pid) {
+    constexpr int CB = Chunks<R>::cb;
     const uint2 e = a.list[qi];
     const int pid = (int)e.x, qid = (int)e.y;
     const int* __restrict__ pps = sizeof(R) == 8 ? a.pair_ps : a.pair_ps32;
@@ -380,7 +389,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             const int orow = SIDE == 0 ? oA : oB;
             sfor<NR>([&](auto r_) {
                 constexpr int r = decltype(r_)::value;
-                constexpr bool has = SIDE == 0 ? chunk_has_a(C0, r) : chunk_has_b(C0, r);
+                constexpr bool has = SIDE == 0 ? chunk_has_a(C0, CB, r) : chunk_has_b(C0, CB, r);
                 if constexpr (has) {
                     if (in_tile) {
                         sfor<NC>([&](auto jc_) {
@@ -475,7 +484,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
         return;
     }
     if (n <= 0) return;
-    unsigned int* ctr = a.chunk_ctr + C0 / CB;
+    unsigned int* ctr = a.chunk_ctr + C0 / Chunks<R>::cb;
     const int n_tiles = a.tile_n > 0 ? (a.Db ? 2 : 1) : 0;
     double* tile = n_tiles ? tile_mem : nullptr;
     load_boys<LTOT, R>(boys_of<R>(a), tab);
@@ -533,7 +542,7 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     const int ch = (int)(blockIdx.x % NC), vb = (int)(blockIdx.x / NC), nvb = (int)(gridDim.x / NC);
     sfor<NC>([&](auto c_) {
         constexpr int c = decltype(c_)::value;
-        if (ch == c) run_chunk<R, (CH0 + c) * CB>(a, vb, nvb, tab, tile_mem, s_chunk);
+        if (ch == c) run_chunk<R, (CH0 + c) * Chunks<R>::cb>(a, vb, nvb, tab, tile_mem, s_chunk);
     });
 }
 
@@ -544,11 +553,16 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
 #define HFX_CHUNK_END 1000
 #endif
 constexpr int CH_BEGIN = HFX_CHUNK_BEGIN;
-constexpr int CH_END = HFX_CHUNK_END < NCH ? HFX_CHUNK_END : NCH;
+// Chunks [CH_BEGIN, CH_END) of this translation unit in the chunking of precision R.
+template <class R>
+constexpr int ch_end() {
+    return HFX_CHUNK_END < Chunks<R>::nch ? HFX_CHUNK_END : Chunks<R>::nch;
+}
 
 template <class R>
-void launch_part(const EriArgs& a, int grid, const hipStream_t* cs) {
-    constexpr int NC = CH_END > CH_BEGIN ? CH_END - CH_BEGIN : 0;
+void launch_part(int fp32, const EriArgs& a, int grid, const hipStream_t* cs) {
+    (void)fp32;
+    constexpr int NC = ch_end<R>() > CH_BEGIN ? ch_end<R>() - CH_BEGIN : 0;
     if constexpr (NC > 0) {
         const size_t lds = a.tile_n > 0 ? (size_t)(a.Db ? 2 : 1) * (NA + NB) * a.tile_n * sizeof(double) +
                                               (size_t)((a.tile_shells + 31) / 32) * sizeof(unsigned int)
@@ -558,14 +572,12 @@ void launch_part(const EriArgs& a, int grid, const hipStream_t* cs) {
     }
 }
 
-void launch_this_part(int fp32, const EriArgs& a, int grid, const hipStream_t* cs) {
-    if (fp32)
-        launch_part<float>(a, grid, cs);
-    else
-        launch_part<double>(a, grid, cs);
-}
-
-const bool registered = register_eri_part(K_COMBO, CH_BEGIN, CH_END, NCH, &launch_this_part);
+const bool registered64 =
+    register_eri_part(K_COMBO, 0, CH_BEGIN, ch_end<double>(), Chunks<double>::nch, &launch_part<double>);
+const bool registered32 = ch_end<float>() > CH_BEGIN
+                              ? register_eri_part(K_COMBO, 1, CH_BEGIN, ch_end<float>(), Chunks<float>::nch,
+                                                  &launch_part<float>)
+                              : false;
 
 }  // namespace
 }  // namespace hfx
