@@ -49,12 +49,16 @@ constexpr int LA = pc_la(PC1), LB = pc_lb(PC1), LC = pc_la(PC2), LD = pc_lb(PC2)
 constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
 constexpr int NAB = NA * NB, NCD = NC * ND;
 // Chunking of the bra components: a thread accumulates at most `acc` integrals (cb bra
-// components x all NCD ket components), so a class is split into nch chunks.  FP32 threads of
-// the classes with L >= 4 hold twice as many (they run at one wavefront per SIMD either way,
-// and fewer chunks repeat the Boys function, R and the E tables fewer times).
+// components x all NCD ket components), so a class is split into nch chunks.  The classes with
+// L >= 4 run at one wavefront per SIMD whatever their accumulators, so their threads hold more
+// (FP32: 72, 108 for L >= 5; FP64: 72): fewer chunks repeat the Boys function, R^0_tuv and the
+// E tables fewer times.
+constexpr int ACC32_HI = 108;
+constexpr int ACC64_HI = 72;
 template <class R>
 struct Chunks {
-    static constexpr int acc = (sizeof(R) == 4 && LA + LB + LC + LD >= 4) ? 72 : 36;
+    static constexpr int acc = sizeof(R) == 4 ? (LA + LB + LC + LD >= 5 ? ACC32_HI : (LA + LB + LC + LD >= 4 ? 72 : 36))
+                                              : (LA + LB + LC + LD >= 4 ? ACC64_HI : 36);
     static constexpr int cb_max = cmax(1, acc / NCD);
     static constexpr int nch = (NAB + cb_max - 1) / cb_max;
     static constexpr int cb = (NAB + nch - 1) / nch;
