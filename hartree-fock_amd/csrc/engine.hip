@@ -203,8 +203,8 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
 struct ScreenClass {
     const int* bra_ids;  // pair ids in sorted order
     int n_bra;           // 0: the class has no work
-    const int* ket_ids;
-    const double* ket_q;  // Q of each ket-list position
+    const int4* ket_info;  // {pair, local a | local b << 16, nprim, 0} of each ket-list position
+    const double* ket_q;   // Q of each ket-list position
     const int* ket_seg_start;
     const int* ket_seg_end;
     const int* ket_molseg;  // [mol] -> first segment, [mol+1] -> end (per ket class)
@@ -321,9 +321,9 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                 int4 pk = make_int4(0, 0, 0, 0);
                 double mx = 0.0;
                 if (x < xend) {
-                    qid = K.ket_ids[x];
-                    pk = a.pair_info[qid];
-                    const int lc = pk.z & 0xffff, ld = pk.z >> 16;
+                    pk = K.ket_info[x];
+                    qid = pk.x;
+                    const int lc = pk.y & 0xffff, ld = pk.y >> 16;
                     mx = fmax(tab2, 2.0 * a.Mt[qb + lc + (long long)ld * ns]);
                     mx = fmax(mx, fmax(fmax(ra[lc], ra[ld]), fmax(rb[lc], rb[ld])));
                     if (a.Mb) {
@@ -348,11 +348,12 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                     else
                         a.list[(long long)(b32 + __popcll(m32 & lt_mask))] = ent;
                     if (f32)
-                        prims32 += (unsigned long long)(npa * pk.w);
+                        prims32 += (unsigned long long)(npa * pk.z);
                     else
-                        prims += (unsigned long long)(npa * pk.w);
+                        prims += (unsigned long long)(npa * pk.z);
                     if constexpr (PASS == 1) {
-                        const int sc = pk.x, sd = pk.y;
+                        const int4 pq = a.pair_info[qid];
+                        const int sc = pq.x, sd = pq.y;
                         // Canonical (reference loop order) form of the quartet for the hash.
                         long long A = max(sa, sb), B = min(sa, sb), C = max(sc, sd), D = min(sc, sd);
                         if (A * (A + 1) / 2 + B < C * (C + 1) / 2 + D) {
@@ -686,6 +687,7 @@ struct hfx_engine {
     long long *d_sh_mbase = nullptr, *d_sh_qbase = nullptr, *d_sh_tbase = nullptr;
     double *d_sh_cf = nullptr, *d_ctr = nullptr, *d_alpha = nullptr, *d_coeff = nullptr, *d_norm = nullptr;
     int* d_cls_ids[N_PAIR_CLASSES] = {};
+    int4* d_cls_info[N_PAIR_CLASSES] = {};
     double* d_cls_q[N_PAIR_CLASSES] = {};
     int *d_seg_start[N_PAIR_CLASSES] = {}, *d_seg_end[N_PAIR_CLASSES] = {}, *d_molseg[N_PAIR_CLASSES] = {};
     double* d_boys64 = nullptr;
@@ -813,6 +815,15 @@ int sort_pairs(hfx_engine* e, const std::vector<double>& qt) {
         };
         int rc;
         if ((rc = put(e->d_cls_ids[pc], ids)) || (rc = put(e->d_cls_q[pc], e->cls_q[pc]))) return rc;
+        {
+            // what the screening needs of each position, in one load: {pair, local a | local b << 16, nprim, 0}
+            std::vector<int4> info(ids.size());
+            for (size_t i = 0; i < ids.size(); ++i) {
+                const int p = ids[i];
+                info[i] = make_int4(p, e->sh_local[e->pair_a[p]] | (e->sh_local[e->pair_b[p]] << 16), e->pair_nprim[p], 0);
+            }
+            if ((rc = put(e->d_cls_info[pc], info))) return rc;
+        }
         // segment arrays can change length between sorts only through nprim runs, which are fixed
         // by the pair set; their capacity is the number of pairs in the class.
         if (!e->d_seg_start[pc]) {
@@ -843,7 +854,7 @@ int sort_pairs(hfx_engine* e, const std::vector<double>& qt) {
             const bool live = !e->cls_ids[p1].empty() && !e->cls_ids[p2].empty();
             c.bra_ids = e->d_cls_ids[p1];
             c.n_bra = live ? (int)e->cls_ids[p1].size() : 0;
-            c.ket_ids = e->d_cls_ids[p2];
+            c.ket_info = e->d_cls_info[p2];
             c.ket_q = e->d_cls_q[p2];
             c.ket_seg_start = e->d_seg_start[p2];
             c.ket_seg_end = e->d_seg_end[p2];
