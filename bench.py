@@ -14,12 +14,15 @@ build (one hfx_jk call, dD = D_conv, D_prev = 0) from device-resident inputs.  T
   5. gradient: RHF nuclear gradient of (H2O)8 / 6-31G**, FP64 and mixed
   6. CPU baseline: the oracle's OpenMP port (reference release flags) on a bounded slice
      of the same build, scaled by the model FLOPs of the slice
+  7. conventional path: the Schwarz-screened integral blocks stored in HBM once, then K
+     graph-replayed FP64 builds digesting them (HBM-bound), and the conventional SCF
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Profiling aid: --save-density PATH stores the converged FP64 density (and stops); a later
   python bench.py --profile-builds --density PATH --cut C --steps K --warmup W
 runs only the W + K mixed builds with per-class HIP-event timing (no SCF), so a profiler's
-per-kernel averages cover exactly the launches the roofline is computed from.
+per-kernel averages cover exactly the launches the roofline is computed from; with --stored
+instead of --cut, the same for W + K builds of the conventional path (store, then stored builds).
 """
 import argparse
 import json
@@ -35,8 +38,11 @@ import numpy as np  # noqa: E402
 from hfx import Basis, autotune, device, geometry, lib  # noqa: E402
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X: FP64 vector / dense MFMA; FP32 dense MFMA
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E
 CLASS_NAMES = ["ssss", "psss", "psps", "ppss", "ppps", "pppp", "dsss", "dsps", "dspp", "dsds", "dpss", "dpps",
                "dppp", "dpds", "dpdp", "ddss", "ddps", "ddpp", "ddds", "dddp", "dddd"]
+_L = {"s": 0, "p": 1, "d": 2}
+CLASS_L = [tuple(_L[c] for c in name) for name in CLASS_NAMES]
 
 
 def log(*a):
@@ -160,6 +166,41 @@ def measured_traffic(group):
     return None if t is None else t["read_bytes"] + t["write_bytes"]
 
 
+def stored_run(L, basis, eng, bufs, Q, steps, warmup, J64, K64, Da, scf_kw):
+    """The conventional path at the same density: store, K graph-replayed stored builds, per-class
+    times, the HBM roofline of the dominant stored-digest group, and the conventional SCF."""
+    t0 = time.perf_counter()
+    nq, nbytes = eng.store_eris(1e-10, Q=Q)
+    t_store = time.perf_counter() - t0
+    flags = lib.HFX_JK_STORED | lib.HFX_JK_NO_STATS | lib.HFX_JK_USE_GRAPH
+    dt, _, _, _ = timed_builds(L, eng, bufs, None, steps, warmup, lib.HFX_FP64, 0.0, Q, flags)
+    Js, Ks = bufs[2].download(), bufs[3].download()
+    _, st, cms, _ = timed_builds(L, eng, bufs, None, steps, 1, lib.HFX_FP64, 0.0, Q,
+                                 lib.HFX_JK_STORED | lib.HFX_JK_CLASS_TIMING)
+    eng.release_eris()
+    # bytes a stored build reads per class: the padded blocks and the list entries
+    groups = []
+    for k in range(21):
+        n_k = st["class_count"][k]
+        if n_k == 0 or cms[k] <= 0:
+            continue
+        la, lb, lc, ld = CLASS_L[k]
+        blk = ((la + 1) * (la + 2) // 2) * ((lb + 1) * (lb + 2) // 2) * ((lc + 1) * (lc + 2) // 2) * \
+            ((ld + 1) * (ld + 2) // 2)
+        groups.append((cms[k], k, 8.0 * ((n_k + 63) // 64 * 64 * blk + n_k)))
+    ms, k, byt = max(groups)
+    achieved = byt / ms * 1e-6  # GB/s
+    roof = dict(bound="hbm", kernel="k_eri %s f64 stored digest" % CLASS_NAMES[k], achieved=achieved,
+                peak=PEAK_HBM_GBPS, unit="GB/s", frac=achieved / PEAK_HBM_GBPS, launch_ms=ms, launch_bytes=byt)
+    rconv = lib.scf_run(L, basis, options=L.default_options(direct=0, **scf_kw))
+    dE = abs(float(np.sum(Da.reshape(-1, order="F") * ((Js - Ks) - (J64 - K64)))))
+    return dict(quartets=nq, stored_bytes=nbytes, store_s=t_store, builds_per_s=steps / dt, ms_per_build=1e3 * dt / steps,
+                read_GBps_build=nbytes / (dt / steps) * 1e-9, build_dE_vs_direct=dE,
+                build_max_dJK_vs_direct=float(max(np.abs(Js - J64).max(), np.abs(Ks - K64).max())),
+                class_ms={CLASS_NAMES[i]: round(cms[i], 3) for i in range(21) if cms[i] > 0}, roofline=roof,
+                scf_wall_s=rconv["time_total_s"], scf_iterations=rconv["iterations"], scf_e=rconv["e_total"])
+
+
 def roofline_of(stmx, cm64, cm32):
     """Roofline of the dominant class kernel group (one class, one precision) of a build."""
     groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
@@ -191,6 +232,14 @@ def profile_builds(args):
     device.synchronize()
     src.free()
     dst.free()
+    if args.stored:
+        nq, nbytes = eng.store_eris(1e-10, Q=Q)
+        _, st, cms, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q,
+                                     lib.HFX_JK_STORED | lib.HFX_JK_CLASS_TIMING)
+        print(json.dumps({"profile_stored": args.steps + args.warmup, "quartets": nq, "stored_bytes": nbytes,
+                          "class_ms": {CLASS_NAMES[i]: round(cms[i], 4) for i in range(21) if cms[i] > 0},
+                          "device_ms": st["device_ms"]}), flush=True)
+        return
     _, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, args.cut, Q)
     roof = roofline_of(stmx, cm64, cm32)
     per_class = {CLASS_NAMES[i]: [round(cm64[i], 4), round(cm32[i], 4)] for i in range(21) if cm64[i] + cm32[i] > 0}
@@ -210,14 +259,15 @@ def main():
     ap.add_argument("--density", default=None)
     ap.add_argument("--cut", type=float, default=None)
     ap.add_argument("--profile-builds", action="store_true")
+    ap.add_argument("--stored", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path (the shell-quartet work "
               "list would be sharded with one all-reduce of J/K per build); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.profile_builds:
-        if not args.density or args.cut is None:
-            print("--profile-builds needs --density and --cut", file=sys.stderr)
+        if not args.density or (args.cut is None and not args.stored):
+            print("--profile-builds needs --density and --cut (or --stored)", file=sys.stderr)
             sys.exit(2)
         profile_builds(args)
         return
@@ -290,6 +340,9 @@ def main():
     roof = roofline_of(stmx, cm64, cm32)
     per_class = {CLASS_NAMES[i]: [round(cm64[i], 3), round(cm32[i], 3)] for i in range(21) if cm64[i] + cm32[i] > 0}
 
+    stored = stored_run(L, basis, eng, bufs, Q, args.steps, args.warmup, J64, K64, Da, scf_kw)
+    log("stored: %s" % stored)
+
     batched = batched_run(L, cut)
     log("batched: %s" % batched)
     grad = gradient_run(L)
@@ -333,6 +386,7 @@ def main():
         "build_max_dJK_mixed": dF_build,
         "class_ms_f64_f32": per_class,
         "class_ms_fp64_build": {CLASS_NAMES[i]: round(c64[i], 3) for i in range(21) if c64[i] > 0},
+        "stored": stored,
         "batched": batched,
         "gradient": grad,
         "roofline": roof,

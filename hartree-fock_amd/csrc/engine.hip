@@ -596,10 +596,11 @@ void launch_one(int pc, const OneArgs& o, hipStream_t s) {
 struct BuildParams {
     double tauS, tauD, cut;
     bool mixed, uhf, timing;
-    bool hash = true;  // the survivor-list hash (statistics only)
+    bool hash = true;     // the survivor-list hash (statistics only)
+    bool stored = false;  // digest the stored integrals (no screening pass)
     bool operator==(const BuildParams& o) const {
         return tauS == o.tauS && tauD == o.tauD && cut == o.cut && mixed == o.mixed && uhf == o.uhf &&
-               timing == o.timing && hash == o.hash;
+               timing == o.timing && hash == o.hash && stored == o.stored;
     }
 };
 
@@ -726,9 +727,37 @@ struct hfx_engine {
     static constexpr int CHUNK_LEN = 2048;   // list entries a block claims at a time
     int max_N = 0;
 
-    ~hfx_engine() {
+    // stored integrals (hfx_store_eris): the Schwarz-screened list of every class, its ranges and
+    // the normalized blocks in the stored-integral layout, one region per class
+    uint2* d_slist = nullptr;
+    long long* d_sranges = nullptr;  // [2 * N_COMBOS]
+    double* d_store = nullptr;
+    long long store_n[N_COMBOS] = {}, store_base[N_COMBOS] = {};
+    long long store_elems = 0;
+    double store_tauS = -1.0;
+    void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key = BuildParams{-1.0, -1.0, -1.0, false, false, false};
+    }
+    void release_store() {
+        if (d_store) drop_graph();  // a captured stored build reads these buffers
+        if (d_slist) (void)hipFree(d_slist);
+        if (d_sranges) (void)hipFree(d_sranges);
+        if (d_store) (void)hipFree(d_store);
+        d_slist = nullptr;
+        d_sranges = nullptr;
+        d_store = nullptr;
+        for (int k = 0; k < N_COMBOS; ++k) store_n[k] = store_base[k] = 0;
+        store_elems = 0;
+        store_tauS = -1.0;
+    }
+
+    ~hfx_engine() {
+        release_store();
+        drop_graph();
         for (hipEvent_t& x : cev)
             if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1336,6 +1365,11 @@ int enqueue_screen(hfx_engine* e, const BuildParams& P, hipStream_t s) {
 // quartets run in FP32; the FP64 share is small but runs on an under-filled machine.
 double chunk_cost(hfx_engine* e, int k, int fp32, bool mixed) {
     const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+    if (e->d_store && fp32 < 0) {  // stored-integral digest: the bytes it reads
+        const int nch = std::max(1, eri_chunks(k, 0));
+        const double blk = (double)ncart_h(pc_la(p1)) * ncart_h(pc_lb(p1)) * ncart_h(pc_la(p2)) * ncart_h(pc_lb(p2));
+        return 1e-6 * (double)e->store_n[k] * (blk + 1.0) / nch + 1.0;
+    }
     const std::vector<int>& b = e->cls_ids[p1];
     const std::vector<int>& c = e->cls_ids[p2];
     if (b.empty() || c.empty()) return 0.0;
@@ -1358,7 +1392,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     const bool uhf = P.uhf, mixed = P.mixed, timing = P.timing;
     const long long ne = e->n_elems;
     n_ev = 0;
-    {
+    if (!P.stored) {
         const int rc = enqueue_screen(e, P, s);
         if (rc) return rc;
     }
@@ -1387,6 +1421,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     ea.tauS = P.tauS;
     ea.out = nullptr;
     ea.out_off = nullptr;
+    if (P.stored) ea.list = e->d_slist;
     const int grid = eri_grid(e);
     hipLaunchKernelGGL(k_fill_u32, dim3(4), dim3(256), 0, s, e->d_chunk_ctr, 2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
     ea.chunk_len = hfx_engine::CHUNK_LEN;
@@ -1395,7 +1430,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     auto tile_cols = [&](int k, int fp32) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
-        const size_t boys = (size_t)md::boys_points(!fp32) * md::boys_slice_nord(L, !fp32) * (fp32 ? 4 : 8);
+        const size_t boys = P.stored ? 16 : (size_t)md::boys_points(!fp32) * md::boys_slice_nord(L, !fp32) * (fp32 ? 4 : 8);
         const size_t rows = (size_t)(uhf ? 2 : 1) * (ncart_h(la) + ncart_h(lb)) * e->max_N * sizeof(double);
         const size_t mask = (size_t)((e->ns_max + 31) / 32) * sizeof(unsigned int);
         return boys + rows + mask + 1024 <= 64 * 1024 ? e->max_N : 0;
@@ -1417,7 +1452,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
                 const int nch = eri_chunks(k, f);
                 const int np = eri_parts(k, f, pb, pe);
                 cs[2 * k + f].assign(std::max(1, nch), s);
-                const double c = chunk_cost(e, k, f, mixed);
+                const double c = chunk_cost(e, k, P.stored ? -1 : f, mixed);
                 for (int x = 0; x < np; ++x)
                     if (pe[x] > pb[x]) items.push_back({c * (pe[x] - pb[x]), 2 * k + f, pb[x], pe[x]});
             }
@@ -1441,7 +1476,9 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         ev_slot[2 * k] = ev_slot[2 * k + 1] = -1;
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
-        ea.range = e->d_ranges + 2 * k;
+        if (P.stored && e->store_n[k] == 0) continue;
+        ea.range = (P.stored ? e->d_sranges : e->d_ranges) + 2 * k;
+        ea.stored = P.stored ? e->d_store + e->store_base[k] : nullptr;
         ea.chunk_ctr = e->d_chunk_ctr + (2 * k) * hfx_engine::CHUNK_SLOTS;
         ea.tile_n = tile_cols(k, 0);
         launch_eri(k, 0, ea, grid, cs[2 * k].data());
@@ -1479,6 +1516,21 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     return HFX_OK;
 }
 
+// Class lists sorted by the Schwarz table Q (the caller's, or the engine's own when NULL).
+int use_q(hfx_engine* e, const double* Q, const char* who) {
+    if (Q) {
+        if (!finite_all(Q, e->n_qelems)) return fail(HFX_ERR_NONFINITE, std::string(who) + ": non-finite Q");
+        if (e->sort_q.size() != (size_t)e->n_qelems ||
+            std::memcmp(e->sort_q.data(), Q, e->n_qelems * sizeof(double)) != 0) {
+            std::vector<double> q(Q, Q + e->n_qelems);
+            return sort_pairs(e, q);
+        }
+    } else if (e->sort_q != e->own_q) {
+        return sort_pairs(e, e->own_q);
+    }
+    return HFX_OK;
+}
+
 int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     if (!e || !args || !args->dD_tot || !args->dD_alpha || !J || !Ka)
         return fail(HFX_ERR_INVALID, "hfx_jk: null argument");
@@ -1486,22 +1538,19 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     if (uhf && !Kb) return fail(HFX_ERR_INVALID, "hfx_jk: dD_beta given without Kb");
     if (args->precision != HFX_FP64 && args->precision != HFX_MIXED)
         return fail(HFX_ERR_INVALID, "hfx_jk: unknown precision");
+    const bool stored = (args->flags & HFX_JK_STORED) != 0;
+    if (stored && !e->d_store)
+        return fail(HFX_ERR_INVALID, "hfx_jk: HFX_JK_STORED without stored integrals (call hfx_store_eris first)");
+    if (stored && args->precision != HFX_FP64)
+        return fail(HFX_ERR_UNSUPPORTED, "hfx_jk: the stored integrals are FP64; HFX_JK_STORED needs HFX_FP64");
     const bool devp = (args->flags & HFX_JK_DEVICE_PTRS) != 0;
     const long long ne = e->n_elems;
     if (!devp) {
         if (!finite_all(args->dD_tot, ne) || !finite_all(args->dD_alpha, ne) || (uhf && !finite_all(args->dD_beta, ne)))
             return fail(HFX_ERR_NONFINITE, "hfx_jk: non-finite density");
     }
-    if (args->Q) {
-        if (!finite_all(args->Q, e->n_qelems)) return fail(HFX_ERR_NONFINITE, "hfx_jk: non-finite Q");
-        if (e->sort_q.size() != (size_t)e->n_qelems ||
-            std::memcmp(e->sort_q.data(), args->Q, e->n_qelems * sizeof(double)) != 0) {
-            std::vector<double> q(args->Q, args->Q + e->n_qelems);
-            int rc = sort_pairs(e, q);
-            if (rc) return rc;
-        }
-    } else if (e->sort_q != e->own_q) {
-        int rc = sort_pairs(e, e->own_q);
+    if (!stored) {  // the stored list does not depend on the class order
+        const int rc = use_q(e, args->Q, "hfx_jk");
         if (rc) return rc;
     }
     HIPCHK(hipSetDevice(e->device));
@@ -1516,6 +1565,13 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
                   args->precision == HFX_MIXED && args->fp32_cut > 0.0, uhf,
                   (args->flags & HFX_JK_CLASS_TIMING) != 0};
     P.hash = st != nullptr && !(args->flags & HFX_JK_NO_STATS);
+    if (stored) {  // no density test, no FP32 split: the stored list as it is
+        P.tauD = 0.0;
+        P.cut = 0.0;
+        P.mixed = false;
+        P.hash = false;
+        P.stored = true;
+    }
     int ev_slot[2 * N_COMBOS];
     int n_ev = 0;
     const bool timing = P.timing;
@@ -1549,7 +1605,7 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     if (uhf) HIPCHK(hipMemcpyAsync(Kb, e->d_Kb, ne * sizeof(double), d2h, s));
     long long rng[4 * N_COMBOS];
     unsigned long long cnt[2 * N_COMBOS], stv[4 * N_COMBOS];
-    const bool want_stats = st && !(args->flags & HFX_JK_NO_STATS);
+    const bool want_stats = st && !(args->flags & HFX_JK_NO_STATS) && !stored;
     if (want_stats) {
         HIPCHK(hipMemcpyAsync(rng, e->d_ranges, sizeof(rng), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(stv, e->d_stats, sizeof(stv), hipMemcpyDeviceToHost, s));
@@ -1583,6 +1639,18 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
             }
             st->model_gflop = flop;
         }
+        if (stored && !(args->flags & HFX_JK_NO_STATS)) {  // the stored list; digest FLOPs only
+            double flop = 0.0;
+            for (int k = 0; k < N_COMBOS; ++k) {
+                const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+                const double dig = 12.0 * ncart_h(pc_la(p1)) * ncart_h(pc_lb(p1)) * ncart_h(pc_la(p2)) * ncart_h(pc_lb(p2));
+                st->class_count[k] = e->store_n[k];
+                st->n_quartets += e->store_n[k];
+                st->class_gflop64[k] = dig * (double)e->store_n[k] * 1e-9;
+                flop += st->class_gflop64[k];
+            }
+            st->model_gflop = flop;
+        }
         if (timing) {
             int prev = 0;
             for (int x = 0; x < 2 * N_COMBOS; ++x) {
@@ -1594,6 +1662,89 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
             }
         }
     }
+    return HFX_OK;
+}
+
+// Stored integrals (IntegralEngine::electronRepulsionTensor, src/IntegralEngine.cpp:635-692): the
+// canonical quartets with Q(a,b)Q(c,d) >= tau_S, found by the screening pass with every density
+// test passing (zero density, tau_D = 0), copied into a list of their own, and their normalized
+// blocks computed once by the store mode of the FP64 class kernels, in the layout the stored
+// digest reads (element-major groups of 64 entries).  Replaces any earlier store.
+int store_impl(hfx_engine* e, const double* Q, double tauS, int64_t* n_out, int64_t* bytes_out) {
+    if (!e) return fail(HFX_ERR_INVALID, "hfx_store_eris: null engine");
+    if (!(tauS >= 0.0) || !std::isfinite(tauS)) return fail(HFX_ERR_INVALID, "hfx_store_eris: bad schwarz_thresh");
+    {
+        const int rc = use_q(e, Q, "hfx_store_eris");
+        if (rc) return rc;
+    }
+    HIPCHK(hipSetDevice(e->device));
+    e->release_store();
+    hipStream_t s = e->stream;
+    const long long ne = e->n_elems;
+    const int g = e->n_cu * 4;
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_Dt, ne, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_Da, ne, 0.0);
+    BuildParams P{tauS, 0.0, 0.0, false, false, false};
+    P.hash = false;
+    {
+        const int rc = enqueue_screen(e, P, s);
+        if (rc) return rc;
+    }
+    long long rng[4 * N_COMBOS];
+    HIPCHK(hipMemcpyAsync(rng, e->d_ranges, sizeof(rng), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long long n_tot = 0, elems = 0, sr[2 * N_COMBOS];
+    for (int k = 0; k < N_COMBOS; ++k) {
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        const long long blk = (long long)ncart_h(pc_la(p1)) * ncart_h(pc_lb(p1)) * ncart_h(pc_la(p2)) * ncart_h(pc_lb(p2));
+        const long long n = rng[2 * k + 1] - rng[2 * k];
+        e->store_n[k] = n;
+        e->store_base[k] = elems;
+        sr[2 * k] = n_tot;
+        sr[2 * k + 1] = n_tot + n;
+        n_tot += n;
+        elems += (n + 63) / 64 * 64 * blk;
+    }
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const double need = (double)elems * sizeof(double) + (double)n_tot * sizeof(uint2);
+    if (need + (1ull << 30) > (double)free_b) {
+        e->release_store();
+        return fail(HFX_ERR_NOMEM, "hfx_store_eris: the stored integrals need " + std::to_string(need / 1e9) +
+                                       " GB, " + std::to_string(free_b / 1e9) + " GB free");
+    }
+    if (hipMalloc(&e->d_slist, std::max<long long>(1, n_tot) * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&e->d_sranges, sizeof(sr)) != hipSuccess ||
+        hipMalloc(&e->d_store, std::max<long long>(1, elems) * sizeof(double)) != hipSuccess) {
+        e->release_store();
+        return fail(HFX_ERR_NOMEM, "hfx_store_eris: hipMalloc failed");
+    }
+    e->store_elems = elems;
+    HIPCHK(hipMemcpyAsync(e->d_sranges, sr, sizeof(sr), hipMemcpyHostToDevice, s));
+    for (int k = 0; k < N_COMBOS; ++k)
+        if (e->store_n[k] > 0)
+            HIPCHK(hipMemcpyAsync(e->d_slist + sr[2 * k], e->d_list + rng[2 * k], e->store_n[k] * sizeof(uint2),
+                                  hipMemcpyDeviceToDevice, s));
+    EriArgs a = eri_args(e);
+    a.list = e->d_slist;
+    a.out_off = nullptr;
+    a.tauS = tauS;
+    for (int k = 0; k < N_COMBOS; ++k) {
+        if (e->store_n[k] == 0) continue;
+        if (eri_chunks(k, 0) < 0) {
+            e->release_store();
+            return fail(HFX_ERR_UNSUPPORTED, "ERI kernels missing for class " + std::to_string(k));
+        }
+        a.range = e->d_sranges + 2 * k;
+        a.out = e->d_store + e->store_base[k];
+        std::vector<hipStream_t> cs(std::max(1, eri_chunks(k, 0)), s);
+        launch_eri(k, 0, a, eri_grid(e), cs.data());
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    e->store_tauS = tauS;
+    if (n_out) *n_out = n_tot;
+    if (bytes_out) *bytes_out = (int64_t)need;
     return HFX_OK;
 }
 
@@ -1941,6 +2092,23 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st);
+}
+
+int hfx_store_eris(hfx_engine* e, const double* Q, double schwarz_thresh, int64_t* n_quartets, int64_t* n_bytes) {
+    try {
+        return store_impl(e, Q, schwarz_thresh, n_quartets, n_bytes);
+    } catch (const std::exception& x) {
+        if (e) e->release_store();
+        return fail(HFX_ERR_NOMEM, std::string("hfx_store_eris: ") + x.what());
+    }
+}
+
+int hfx_release_eris(hfx_engine* e) {
+    if (!e) return fail(HFX_ERR_INVALID, "hfx_release_eris: null engine");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->release_store();
+    return HFX_OK;
 }
 
 int hfx_rhf_gradient(hfx_engine* e, const hfx_grad_args* args, double* grad, double* parts, hfx_grad_stats* stats) {

@@ -79,7 +79,10 @@ struct EriArgs {
     int n_copies;
     double tauS;
     double* out;               // store mode (non-NULL): blocks [ab][cd], canonical class order
-    const long long* out_off;  // store mode: offset of each entry's block
+    const long long* out_off;  // store mode: offset of each entry's block (NULL: the stored-integral
+                               // layout of the class list, element-major in groups of 64 entries)
+    const double* stored;      // non-NULL: digest the class's stored integrals (FP64) instead of
+                               // computing them; same layout as the store mode without offsets
     const double* boys64;
     const float* boys32;
     // Chunked schedule (NULL: grid-stride over the list): blocks claim chunk_len consecutive

@@ -400,6 +400,24 @@ struct Scf {
 
 thread_local std::string g_scf_err;
 
+// Conventional mode (DIRECT false, src/SCF.cpp:57-60, 159): every build from the full density
+// through the integrals stored once per run; they are FP64, so the builds are too.
+void conventional_options(hfx_scf_options& o) {
+    if (o.direct) return;
+    o.precision = HFX_FP64;
+    o.final_fp64 = 0;
+    o.full_builds = 1;
+    o.jk_flags |= HFX_JK_STORED;
+}
+
+// Frees the engine's stored integrals when the run that stored them ends.
+struct StoreGuard {
+    hfx_engine* e = nullptr;
+    ~StoreGuard() {
+        if (e) (void)hfx_release_eris(e);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -429,6 +447,7 @@ void hfx_scf_default_options(hfx_scf_options* o) {
     o->final_fp64 = 1;
     o->jk_flags = 0;
     o->full_builds = 0;
+    o->direct = 1;
 }
 
 int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz, int32_t charge,
@@ -441,6 +460,7 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
         s.o = *opt;
     else
         hfx_scf_default_options(&s.o);
+    conventional_options(s.o);
     int64_t N = 0, nS = 0, nmol = 0, elems = 0;
     int rc = hfx_sizes(e, &N, &nS, &nmol, &elems);
     if (rc) return rc;
@@ -456,6 +476,11 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
     if ((rc = hfx_kinetic(e, T.data())) || (rc = hfx_nuclear(e, n_atoms, Z, xyz, nullptr, V.data())) ||
         (rc = hfx_overlap(e, s.S.data())) || (rc = hfx_schwarz(e, s.Q.data())))
         return rc;
+    StoreGuard store;
+    if (!s.o.direct) {  // IntegralEngine::electronRepulsionTensor (SCF.cpp:159)
+        if ((rc = hfx_store_eris(e, s.Q.data(), s.o.schwarz_thresh, nullptr, nullptr))) return rc;
+        store.e = e;
+    }
     s.h = s.mat();
     for (size_t x = 0; x < s.h.size(); ++x) s.h[x] = T[x] + V[x];
     s.X = inv_sqrt(s.n, s.S);
@@ -526,6 +551,7 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
         o = *opt;
     else
         hfx_scf_default_options(&o);
+    conventional_options(o);
 
     // Per-molecule atoms (contiguous runs of atom_mol) and packed block offsets.
     std::vector<int> atom_begin(n_mol + 1, 0);
@@ -569,6 +595,11 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
             s.X = inv_sqrt(s.n, s.S);
             s.initial_guess();
         }
+    }
+    StoreGuard store;
+    if (!o.direct) {  // the engine's own Schwarz table, as the batched builds use
+        if ((rc = hfx_store_eris(e, nullptr, o.schwarz_thresh, nullptr, nullptr))) return rc;
+        store.e = e;
     }
     const auto t_init = clk::now();
 

@@ -107,9 +107,19 @@ __host__ __device__ constexpr bool chunk_has_b(int c0, int cb, int ib) {
     return false;
 }
 
+// Stored blocks (the stored-integral mode): the block of the r-th entry of a class list sits in
+// a group of 64 entries, element-major: element ab*NCD + cd of entry r at
+// ((r / 64) * NAB*NCD + ab*NCD + cd) * 64 + r % 64, so the 64 lanes of a wavefront, which hold 64
+// consecutive entries, read every element with one contiguous 512-byte access.
+__device__ __forceinline__ long long stored_at(long long r, int e) {
+    return ((r >> 6) * (long long)(NAB * NCD) + e) * 64 + (r & 63);
+}
+
 // tile: the block's LDS exchange rows (NA + NB rows of a.tile_n, then the same for K_beta),
 // holding bra pair tile_pid; NULL when the block keeps no rows.
-template <class R, int C0>
+// STORED: the integrals are read from a.stored (normalized, written by the store mode) instead
+// of being computed; no Boys table.
+template <class R, int C0, bool STORED>
 __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab, double* tile,
                                         int tile_pid) {
     constexpr int CB = Chunks<R>::cb;
@@ -117,8 +127,8 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
     const int pid = (int)e.x, qid = (int)e.y;
     const int* __restrict__ pps = sizeof(R) == 8 ? a.pair_ps : a.pair_ps32;
     const int* __restrict__ pnp = sizeof(R) == 8 ? a.pair_np : a.pair_np32;
-    const int ps = pps[pid], np = valid ? pnp[pid] : 0;
-    const int qs = pps[qid], nq = valid ? pnp[qid] : 0;
+    const int ps = STORED ? 0 : pps[pid], np = (valid && !STORED) ? pnp[pid] : 0;
+    const int qs = STORED ? 0 : pps[qid], nq = (valid && !STORED) ? pnp[qid] : 0;
     const PrimPair<R>* __restrict__ pp = pp_of<R>(a);
 
     R acc[CB][NCD];
@@ -263,21 +273,43 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
 
     const int sa = a.pair_a[pid], sb = a.pair_b[pid], sc = a.pair_a[qid], sd = a.pair_b[qid];
     double cfa[NA], cfb[NB], cfc[NC], cfd[ND];
-    sfor<NA>([&](auto x) { cfa[x] = a.sh_cf[6 * sa + x]; });
-    sfor<NB>([&](auto x) { cfb[x] = a.sh_cf[6 * sb + x]; });
-    sfor<NC>([&](auto x) { cfc[x] = a.sh_cf[6 * sc + x]; });
-    sfor<ND>([&](auto x) { cfd[x] = a.sh_cf[6 * sd + x]; });
+    if constexpr (STORED) {
+        // the stored integrals are normalized already
+        sfor<NA>([&](auto x) { cfa[x] = 1.0; });
+        sfor<NB>([&](auto x) { cfb[x] = 1.0; });
+        sfor<NC>([&](auto x) { cfc[x] = 1.0; });
+        sfor<ND>([&](auto x) { cfd[x] = 1.0; });
+        if (valid) {
+            const double* __restrict__ src = a.stored + stored_at(qi - a.range[0], 0);
+            sfor<CB>([&](auto c_) {
+                constexpr int ab = C0 + decltype(c_)::value;
+                if constexpr (ab < NAB) {
+                    sfor<NCD>([&](auto cd_) {
+                        constexpr int cd = decltype(cd_)::value;
+                        acc[decltype(c_)::value][cd] = __builtin_nontemporal_load(&src[(ab * NCD + cd) * 64]);
+                    });
+                }
+            });
+        }
+    } else {
+        sfor<NA>([&](auto x) { cfa[x] = a.sh_cf[6 * sa + x]; });
+        sfor<NB>([&](auto x) { cfb[x] = a.sh_cf[6 * sb + x]; });
+        sfor<NC>([&](auto x) { cfc[x] = a.sh_cf[6 * sc + x]; });
+        sfor<ND>([&](auto x) { cfd[x] = a.sh_cf[6 * sd + x]; });
+    }
 
-    if (a.out) {  // store mode: normalized block, no screening
+    if (!STORED && a.out) {  // store mode: normalized block, no screening
         if (!valid) return;
-        double* out = a.out + a.out_off[qi];
+        // at the entry's offset, or in the stored-integral layout when no offsets are given
+        double* out = a.out_off ? a.out + a.out_off[qi] : a.out + stored_at(qi - a.range[0], 0);
+        const int es = a.out_off ? 1 : 64;
         sfor<CB>([&](auto c_) {
             constexpr int ab = C0 + decltype(c_)::value;
             if constexpr (ab < NAB) {
                 sfor<NCD>([&](auto cd_) {
                     constexpr int cd = decltype(cd_)::value;
-                    out[ab * NCD + cd] = (double)acc[decltype(c_)::value][cd] * (cfa[ab / NB] * cfb[ab % NB]) *
-                                         (cfc[cd / ND] * cfd[cd % ND]);
+                    out[(ab * NCD + cd) * es] = (double)acc[decltype(c_)::value][cd] * (cfa[ab / NB] * cfb[ab % NB]) *
+                                                (cfc[cd / ND] * cfd[cd % ND]);
                 });
             }
         });
@@ -467,7 +499,7 @@ __device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles)
 }
 
 // The work of one chunk kernel on virtual block vb of nvb (several chunks share a launch).
-template <class R, int C0>
+template <class R, int C0, bool STORED>
 __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* tab, double* tile_mem,
                                           long long& s_chunk) {
     const long long lo = a.range[0], hi = a.range[1];
@@ -475,7 +507,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
     if (a.chunk_ctr == nullptr) {
         const long long first = (long long)vb * blockDim.x;
         if (first >= n) return;  // the whole block has no work: uniform exit before the barrier
-        load_boys<LTOT, R>(boys_of<R>(a), tab);
+        if constexpr (!STORED) load_boys<LTOT, R>(boys_of<R>(a), tab);
         __syncthreads();
         // Grid-stride over the list; the trip count is block-uniform, so every lane reaches
         // the wavefront reductions of the digest.
@@ -483,7 +515,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
         for (long long base = first; base < n; base += stride) {
             const long long w = base + threadIdx.x;
             const bool valid = w < n;
-            quartet<R, C0>(a, lo + (valid ? w : 0), valid, tab, nullptr, -1);
+            quartet<R, C0, STORED>(a, lo + (valid ? w : 0), valid, tab, nullptr, -1);
         }
         return;
     }
@@ -491,7 +523,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
     unsigned int* ctr = a.chunk_ctr + C0 / Chunks<R>::cb;
     const int n_tiles = a.tile_n > 0 ? (a.Db ? 2 : 1) : 0;
     double* tile = n_tiles ? tile_mem : nullptr;
-    load_boys<LTOT, R>(boys_of<R>(a), tab);
+    if constexpr (!STORED) load_boys<LTOT, R>(boys_of<R>(a), tab);
     for (int x = threadIdx.x; x < n_tiles * (NA + NB) * a.tile_n; x += blockDim.x) tile_mem[x] = 0.0;
     if (tile)
         for (int w = threadIdx.x; w < (a.tile_shells + 31) / 32; w += blockDim.x)
@@ -525,7 +557,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
             }
             const long long w = base + threadIdx.x;
             const bool valid = w < c1;
-            quartet<R, C0>(a, lo + (valid ? w : c0), valid, tab, tile, tile_pid);
+            quartet<R, C0, STORED>(a, lo + (valid ? w : c0), valid, tab, tile, tile_pid);
         }
     }
     if (tile && tile_pid >= 0) {
@@ -537,16 +569,17 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
 // Chunks [CH0, CH0 + NC) of the class in one launch: block b works on chunk b % NC as
 // virtual block b / NC, so the chunks are interleaved over the machine and run side by side.
 // The class index is a template argument only so that profiles name each class's kernels.
-template <int KC, class R, int CH0, int NC>
+// STORED: the FP64 stored-integral digest (no Boys table in LDS).
+template <int KC, class R, int CH0, int NC, bool STORED>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     using S = BoysSlice<LTOT, R>;
-    __shared__ __align__(16) R tab[BoysGrid<R>::points * S::nord];
+    __shared__ __align__(16) R tab[STORED ? 2 : BoysGrid<R>::points * S::nord];
     __shared__ long long s_chunk;
     extern __shared__ double tile_mem[];
     const int ch = (int)(blockIdx.x % NC), vb = (int)(blockIdx.x / NC), nvb = (int)(gridDim.x / NC);
     sfor<NC>([&](auto c_) {
         constexpr int c = decltype(c_)::value;
-        if (ch == c) run_chunk<R, (CH0 + c) * Chunks<R>::cb>(a, vb, nvb, tab, tile_mem, s_chunk);
+        if (ch == c) run_chunk<R, (CH0 + c) * Chunks<R>::cb, STORED>(a, vb, nvb, tab, tile_mem, s_chunk);
     });
 }
 
@@ -572,7 +605,13 @@ void launch_part(int fp32, const EriArgs& a, int grid, const hipStream_t* cs) {
                                               (size_t)((a.tile_shells + 31) / 32) * sizeof(unsigned int)
                                         : 0;
         const int blocks = (grid > NC ? grid / NC : 1) * NC;
-        hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC>), dim3(blocks), dim3(256), lds, cs[CH_BEGIN], a);
+        if constexpr (sizeof(R) == 8) {
+            if (a.stored) {
+                hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, true>), dim3(blocks), dim3(256), lds, cs[CH_BEGIN], a);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, false>), dim3(blocks), dim3(256), lds, cs[CH_BEGIN], a);
     }
 }
 

@@ -19,6 +19,7 @@ HFX_JK_DEVICE_PTRS = 1
 HFX_JK_USE_GRAPH = 2
 HFX_JK_NO_STATS = 4
 HFX_JK_CLASS_TIMING = 8
+HFX_JK_STORED = 16
 N_CLASSES = 21
 
 ERRORS = {1: "invalid argument", 2: "unsupported", 3: "out of memory", 4: "device error",
@@ -74,7 +75,7 @@ class HfxScfOptions(C.Structure):
                 ("stop_damp_thresh", C.c_double), ("level_shift", C.c_double), ("lshift_gap_tol", C.c_double),
                 ("max_lshift_iter", C.c_int32), ("stop_lshift_thresh", C.c_double), ("precision", C.c_int32),
                 ("fp32_cut", C.c_double), ("final_fp64", C.c_int32), ("jk_flags", C.c_int32),
-                ("full_builds", C.c_int32)]
+                ("full_builds", C.c_int32), ("direct", C.c_int32)]
 
 
 class HfxScfResult(C.Structure):
@@ -92,7 +93,7 @@ class HfxScfResult(C.Structure):
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
                "hfx_nuclear", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk", "hfx_last_error",
                "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine", "hfx_scf_run_batched",
-               "hfx_scf_run_batched_engine", "hfx_rhf_gradient"]
+               "hfx_scf_run_batched_engine", "hfx_rhf_gradient", "hfx_store_eris", "hfx_release_eris"]
 
 
 def _p(a, t=_f64p):
@@ -122,6 +123,8 @@ class Library:
         d.hfx_nuclear.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _i32p, _f64p]
         d.hfx_eri_quartets.argtypes = [C.c_void_p, C.c_int64, _i32p, C.POINTER(C.c_int64), _f64p]
         d.hfx_jk.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.POINTER(HfxJkStats)]
+        d.hfx_store_eris.argtypes = [C.c_void_p, _f64p, C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        d.hfx_release_eris.argtypes = [C.c_void_p]
         d.hfx_rhf_gradient.argtypes = [C.c_void_p, C.POINTER(HfxGradArgs), _f64p, _f64p, C.POINTER(HfxGradStats)]
         d.hfx_scf_default_options.argtypes = [C.POINTER(HfxScfOptions)]
         d.hfx_scf_default_options.restype = None
@@ -255,6 +258,22 @@ class Engine:
         for (a, b, c, d), o, s in zip(q, offs, sizes):
             res.append(out[o:o + s].reshape(sh[a].nao, sh[b].nao, sh[c].nao, sh[d].nao))
         return res
+
+    def store_eris(self, schwarz_thresh=1e-10, Q=None):
+        """Compute and keep in HBM the integral blocks of every shell quartet passing the Schwarz
+        test (IntegralEngine::electronRepulsionTensor, src/IntegralEngine.cpp:635-692); later
+        jk(..., flags=HFX_JK_STORED) builds digest them.  Returns (stored quartets, bytes)."""
+        q = None
+        if Q is not None:
+            qs = Q if isinstance(Q, (list, tuple)) else [Q]
+            q = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1, order="F") for x in qs]))
+        n, b = C.c_int64(0), C.c_int64(0)
+        self.lib.check(self.lib.dll.hfx_store_eris(self.h, _p(q), schwarz_thresh, C.byref(n), C.byref(b)),
+                       "hfx_store_eris")
+        return n.value, b.value
+
+    def release_eris(self):
+        self.lib.check(self.lib.dll.hfx_release_eris(self.h), "hfx_release_eris")
 
     def jk(self, dD_tot, dD_alpha, dD_beta=None, Q=None, schwarz_thresh=1e-10, density_thresh=1e-10,
            precision=HFX_FP64, fp32_cut=0.0, flags=0):

@@ -39,7 +39,9 @@ enum {
     HFX_JK_DEVICE_PTRS = 1, /* density inputs and J/K outputs are device pointers           */
     HFX_JK_USE_GRAPH = 2,   /* replay the engine's captured hipGraph of the build pipeline   */
     HFX_JK_NO_STATS = 4,    /* skip the survivor statistics read-back                         */
-    HFX_JK_CLASS_TIMING = 8 /* time every class kernel group with HIP events (stats.class_ms*) */
+    HFX_JK_CLASS_TIMING = 8, /* time every class kernel group with HIP events (stats.class_ms*) */
+    HFX_JK_STORED = 16       /* digest the integrals kept by hfx_store_eris: no density test, FP64 only;
+                              * the conventional build SCF::buildFockMatrix (src/SCF.cpp:233-521) */
 };
 
 #define HFX_MAX_L 2
@@ -145,6 +147,21 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
  * F = F_prev + J - K (src/SCF.cpp:731).  Kb may be NULL for RHF.  stats may be NULL. */
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb,
            hfx_jk_stats* stats);
+
+/* Conventional (stored-integral) mode.  Computes once, and keeps in HBM, the normalized
+ * integral blocks of every canonical shell quartet with Q(a,b)Q(c,d) >= schwarz_thresh (Q as
+ * in hfx_jk_args: the caller's table or, when NULL, the engine's own).  Replaces
+ * IntegralEngine::electronRepulsionTensor (include/IntegralEngine.hpp:30,
+ * src/IntegralEngine.cpp:635-692), which fills the reference's ElectronRepulsionTensor
+ * (include/Molecule.hpp:10-80); here only the surviving blocks are kept, not the dense packed
+ * tensor.  Afterwards hfx_jk with HFX_JK_STORED digests them against any density (the
+ * conventional build of src/SCF.cpp:233-521: no density test, the AO test |eri| < tau_S with the
+ * build's schwarz_thresh).  Fails with HFX_ERR_NOMEM when the blocks do not fit in the free
+ * device memory.  Outputs the number of stored quartets and the bytes held (either may be NULL).
+ * A later call replaces the store; hfx_release_eris (or hfx_destroy) frees it. */
+int hfx_store_eris(hfx_engine* e, const double* Q, double schwarz_thresh, int64_t* n_quartets,
+                   int64_t* n_bytes);
+int hfx_release_eris(hfx_engine* e);
 
 /* RHF nuclear gradient inputs.  No reference counterpart: the reference computes energies
  * only (src/SCF.cpp:23-136); this is the analytic derivative of that energy,

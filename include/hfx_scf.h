@@ -43,6 +43,9 @@ typedef struct hfx_scf_options {
                                /* and iterate to convergence again                         */
     int32_t jk_flags;          /* extra HFX_JK_* flags for every build                      */
     int32_t full_builds;       /* 1: every build from the full density (no increments)      */
+    int32_t direct;            /* DIRECT             1: direct builds; 0: conventional, the  */
+                               /* integrals stored once (hfx_store_eris) and every build from */
+                               /* the full density through them (FP64; src/SCF.cpp:57-60)     */
 } hfx_scf_options;
 
 typedef struct hfx_scf_result {

@@ -24,6 +24,7 @@
 #include "../include/hfx.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -238,6 +239,10 @@ struct Mol {
     std::vector<int> cart;  // 3 x N
     std::vector<double> Q;  // nS x nS, filled on demand
     bool haveQ = false;
+    // hfx_store_eris: the unique integrals in the reference's packed canonical order
+    // (ElectronRepulsionTensor, include/Molecule.hpp:10-80) and the Schwarz survivors counted
+    std::vector<double> vee;
+    int64_t stored_quartets = 0;
 };
 
 }  // namespace
@@ -247,6 +252,7 @@ struct hfx_engine {
     std::vector<int64_t> mat_off;  // packed N_m x N_m block offsets
     std::vector<int64_t> shell_off;
     int64_t nS_tot = 0, N_tot = 0, elems = 0;
+    bool stored = false;
 };
 
 namespace {
@@ -1087,6 +1093,98 @@ void grad_nuc(int nat, const double* Z, const double* xyz, double* g) {
 
 }  // namespace
 
+namespace {
+
+// Index of the unique element (ij|kl) in the packed canonical tensor (include/Molecule.hpp:42-56).
+inline size_t vee_index(size_t i, size_t j, size_t k, size_t l) {
+    if (j > i) std::swap(i, j);
+    if (l > k) std::swap(k, l);
+    size_t I = i * (i + 1) / 2 + j, K = k * (k + 1) / 2 + l;
+    if (K > I) std::swap(I, K);
+    return I * (I + 1) / 2 + K;
+}
+
+// IntegralEngine::electronRepulsionTensor (IntegralEngine.cpp:635-692): the diagonal quartets
+// (i j|i j) of every shell pair, then every other canonical quartet with Q(i,j)Q(k,l) >= tau.
+// Each block comes from eri_quartet with the reference's zero rules, whose surviving elements
+// are exactly the unique AO quartets the shell quartet owns (IntegralEngine.cpp:600-626).
+int store_tensor(Mol& m, const double* Qext, double tau) {
+    const int nS = m.nS, N = m.N;
+    const size_t T = (size_t)N * (N + 1) / 2;
+    if (T * (T + 1) / 2 > ((size_t)1 << 31)) return fail(HFX_ERR_NOMEM, "oracle hfx_store_eris: tensor too large");
+    const double* Q = Qext;
+    if (!Q) {
+        schwarz(m);
+        Q = m.Q.data();
+    }
+    m.vee.assign(T * (T + 1) / 2, 0.0);
+    std::vector<std::array<int, 4>> qs;
+    int64_t counted = 0;
+    for (int i = 0; i < nS; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int k = 0; k <= i; ++k)
+                for (int l = 0; l <= k; ++l) {
+                    if (i * (i + 1) / 2 + j < k * (k + 1) / 2 + l) continue;
+                    const bool pass = Q[i + (size_t)j * nS] * Q[k + (size_t)l * nS] >= tau;
+                    counted += pass;
+                    if (pass || (i == k && j == l)) qs.push_back({i, j, k, l});
+                }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (size_t x = 0; x < qs.size(); ++x) {
+        const int a = qs[x][0], b = qs[x][1], c = qs[x][2], d = qs[x][3];
+        const int nA = m.nao[a], nB = m.nao[b], nC = m.nao[c], nD = m.nao[d];
+        std::vector<double> G((size_t)nA * nB * nC * nD, 0.0);
+        eri_quartet(m, a, b, c, d, G.data());
+        for (int p = 0; p < nA; ++p)
+            for (int q = 0; q < nB; ++q)
+                for (int r = 0; r < nC; ++r)
+                    for (int t = 0; t < nD; ++t) {
+                        const int i = m.aoff[a] + p, j = m.aoff[b] + q, k = m.aoff[c] + r, l = m.aoff[d] + t;
+                        if (j > i || l > k) continue;
+                        if (a == c && b == d && i * (i + 1) / 2 + j < k * (k + 1) / 2 + l) continue;
+                        // every unique element belongs to one shell quartet: no two threads write it
+                        m.vee[vee_index(i, j, k, l)] = G[(((size_t)p * nB + q) * nC + r) * nD + t];
+                    }
+    }
+    m.stored_quartets = counted;
+    return HFX_OK;
+}
+
+// SCF::buildFockMatrix (SCF.cpp:233-521) from the stored tensor: every unique element
+// (ij|kl), i >= j, k >= l, ij >= kl, with |eri| >= tau, digested into all of its distinct index
+// permutations: J(p,q) += (pq|rs) D_tot(r,s), K(p,s) += (pq|rs) D_x(q,r), each of the 8
+// permutations weighted by 1/(number of coincident ones).  Full symmetric outputs.
+void stored_jk(const Mol& m, const double* Dt, const double* Da, const double* Db, double tau, double* J,
+               double* Ka, double* Kb) {
+    const int N = m.N;
+    std::fill(J, J + (size_t)N * N, 0.0);
+    std::fill(Ka, Ka + (size_t)N * N, 0.0);
+    if (Db) std::fill(Kb, Kb + (size_t)N * N, 0.0);
+    auto at = [N](int r, int c) { return r + (size_t)c * N; };
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int k = 0; k <= i; ++k)
+                for (int l = 0; l <= k; ++l) {
+                    if (i * (i + 1) / 2 + j < k * (k + 1) / 2 + l) continue;
+                    const double v0 = m.vee[vee_index(i, j, k, l)];
+                    if (std::abs(v0) < tau) continue;
+                    double w = 1.0;
+                    if (i == j) w *= 0.5;
+                    if (k == l) w *= 0.5;
+                    if (i == k && j == l) w *= 0.5;
+                    const double v = v0 * w;
+                    const int P[8][4] = {{i, j, k, l}, {j, i, k, l}, {i, j, l, k}, {j, i, l, k},
+                                         {k, l, i, j}, {l, k, i, j}, {k, l, j, i}, {l, k, j, i}};
+                    for (const auto& p : P) {
+                        J[at(p[0], p[1])] += v * Dt[at(p[2], p[3])];
+                        Ka[at(p[0], p[3])] += v * Da[at(p[1], p[2])];
+                        if (Db) Kb[at(p[0], p[3])] += v * Db[at(p[1], p[2])];
+                    }
+                }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* hfx_last_error(void) { return g_err.c_str(); }
@@ -1201,6 +1299,18 @@ static int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka
         (args->dD_beta && !all_finite(args->dD_beta, e->elems)))
         return fail(HFX_ERR_NONFINITE, "hfx_jk: non-finite density");
     if (st) std::memset(st, 0, sizeof(*st));
+    if (args->flags & HFX_JK_STORED) {
+        if (!e->stored) return fail(HFX_ERR_INVALID, "hfx_jk: HFX_JK_STORED without stored integrals");
+        if (args->precision != HFX_FP64) return fail(HFX_ERR_UNSUPPORTED, "hfx_jk: HFX_JK_STORED needs HFX_FP64");
+        for (size_t im = 0; im < e->mols.size(); ++im) {
+            const Mol& m = e->mols[im];
+            const int64_t o = e->mat_off[im];
+            stored_jk(m, args->dD_tot + o, args->dD_alpha + o, args->dD_beta ? args->dD_beta + o : nullptr,
+                      args->schwarz_thresh, J + o, Ka + o, args->dD_beta ? Kb + o : nullptr);
+            if (st) st->n_quartets += m.stored_quartets;
+        }
+        return HFX_OK;
+    }
     int64_t qoff = 0;
     for (size_t im = 0; im < e->mols.size(); ++im) {
         Mol& m = e->mols[im];
@@ -1217,6 +1327,35 @@ static int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st, 0, 1 << 30);
+}
+
+int hfx_store_eris(hfx_engine* e, const double* Q, double schwarz_thresh, int64_t* n_quartets, int64_t* n_bytes) {
+    if (!e) return fail(HFX_ERR_INVALID, "hfx_store_eris: null engine");
+    if (!(schwarz_thresh >= 0.0) || !std::isfinite(schwarz_thresh))
+        return fail(HFX_ERR_INVALID, "hfx_store_eris: bad schwarz_thresh");
+    e->stored = false;
+    int64_t n = 0, bytes = 0, qoff = 0;
+    for (Mol& m : e->mols) {
+        const int rc = store_tensor(m, Q ? Q + qoff : nullptr, schwarz_thresh);
+        if (rc) {
+            for (Mol& x : e->mols) std::vector<double>().swap(x.vee);
+            return rc;
+        }
+        n += m.stored_quartets;
+        bytes += (int64_t)(m.vee.size() * sizeof(double));
+        qoff += (int64_t)m.nS * m.nS;
+    }
+    e->stored = true;
+    if (n_quartets) *n_quartets = n;
+    if (n_bytes) *n_bytes = bytes;
+    return HFX_OK;
+}
+
+int hfx_release_eris(hfx_engine* e) {
+    if (!e) return fail(HFX_ERR_INVALID, "hfx_release_eris: null engine");
+    for (Mol& m : e->mols) std::vector<double>().swap(m.vee);
+    e->stored = false;
+    return HFX_OK;
 }
 
 int hfx_rhf_gradient(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts, hfx_grad_stats* st) {

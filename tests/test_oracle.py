@@ -421,3 +421,69 @@ def test_gradient_invariances(oracle):
     np.fill_diagonal(r3, np.inf)
     gn = -(Z[:, None, None] * Z[None, :, None] * d / r3[:, :, None]).sum(1)
     assert np.abs(parts[0] - gn).max() <= 1e-12
+
+
+@pytest.mark.parametrize("mol", ["h2o_631gss", "oh_uhf"])
+def test_stored_jk_against_bruteforce(oracle, mol):
+    """The conventional path: the packed canonical tensor of hfx_store_eris digested by
+    hfx_jk(HFX_JK_STORED) equals einsum over the dense tensor of the quartet blocks, and the
+    direct build with the density test off."""
+    rng = np.random.default_rng(12)
+    b = molecules.h2o("6-31g**") if mol == "h2o_631gss" else molecules.oh_radical()
+    e = lib.Engine(oracle, b)
+    G = os_reference.dense_eri(e, b)
+    n = b.n_aos
+    Da = _random_density(n, rng, 5)
+    Db = _random_density(n, rng, 4) if mol == "oh_uhf" else None
+    Dt = Da + (Db if Db is not None else Da)
+    nq, nbytes = e.store_eris(0.0)
+    T = n * (n + 1) // 2
+    assert nbytes == 8 * T * (T + 1) // 2
+    J, Ka, Kb, st = e.jk(Dt, Da, Db, schwarz_thresh=0.0, flags=lib.HFX_JK_STORED)
+    assert st["n_quartets"] == nq
+    assert np.abs(J - np.einsum("ijkl,kl->ij", G, Dt)).max() < 1e-12
+    assert np.abs(Ka - np.einsum("ijkl,jk->il", G, Da)).max() < 1e-12
+    if Db is not None:
+        assert np.abs(Kb - np.einsum("ijkl,jk->il", G, Db)).max() < 1e-12
+    # with the thresholds on: the direct build without density test gives the same matrices
+    e.store_eris(1e-10)
+    J, Ka, Kb, _ = e.jk(Dt, Da, Db, schwarz_thresh=1e-10, flags=lib.HFX_JK_STORED)
+    Jd, Kad, Kbd, _ = e.jk(Dt, Da, Db, schwarz_thresh=1e-10, density_thresh=0.0)
+    assert np.abs(J - Jd).max() < 1e-12 and np.abs(Ka - Kad).max() < 1e-12
+    if Db is not None:
+        assert np.abs(Kb - Kbd).max() < 1e-12
+
+
+def test_stored_count_and_errors(oracle):
+    """Stored quartets = canonical quartets passing Q(a,b)Q(c,d) >= tau; the stored mode
+    refuses to run without a store or in mixed precision."""
+    b = molecules.water_dimer("6-31g")
+    e = lib.Engine(oracle, b)
+    Q = e.schwarz()
+    nS = b.n_shells
+    tau = 1e-6
+    want = sum(1 for a in range(nS) for bb in range(a + 1) for c in range(nS) for d in range(c + 1)
+               if a * (a + 1) // 2 + bb >= c * (c + 1) // 2 + d and Q[a, bb] * Q[c, d] >= tau)
+    D = np.eye(b.n_aos) * 0.1
+    with pytest.raises(lib.HfxError):
+        e.jk(D, D, flags=lib.HFX_JK_STORED)
+    nq, _ = e.store_eris(tau)
+    assert nq == want
+    with pytest.raises(lib.HfxError):
+        e.jk(D, D, flags=lib.HFX_JK_STORED, precision=lib.HFX_MIXED, fp32_cut=1e-2)
+    e.release_eris()
+    with pytest.raises(lib.HfxError):
+        e.jk(D, D, flags=lib.HFX_JK_STORED)
+
+
+def test_conventional_scf_matches_direct(oracle):
+    """DIRECT false (stored integrals, every build from the full density) converges to the energy
+    of the dense-tensor SCF, as the direct driver does."""
+    b = molecules.h2o("6-31g**")
+    opts = dict(energy_tol=1e-12, density_tol=1e-10, diis_error_tol=1e-10, max_iter=100,
+                schwarz_thresh=1e-14, density_thresh=1e-14)
+    rc = lib.scf_run(oracle, b, options=oracle.default_options(direct=0, **opts))
+    rd = lib.scf_run(oracle, b, options=oracle.default_options(direct=1, full_builds=1, **opts))
+    assert rc["converged"] == 1 and rd["converged"] == 1
+    assert rc["e_total"] == pytest.approx(rd["e_total"], abs=1e-10)
+    assert rc["iterations"] == rd["iterations"]
