@@ -13,6 +13,12 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libhfx_mi355x.so")
 
 HFX_OK = 0
+HFX_ERR_INVALID = 1
+HFX_ERR_UNSUPPORTED = 2
+HFX_ERR_NOMEM = 3
+HFX_ERR_DEVICE = 4
+HFX_ERR_NONFINITE = 5
+HFX_ERR_NOT_CONVERGED = 6
 HFX_FP64 = 0
 HFX_MIXED = 1
 HFX_JK_DEVICE_PTRS = 1

@@ -39,3 +39,10 @@ def test_missing_library_fails_loudly(tmp_path):
 def test_oracle_exports_the_same_abi(oracle):
     for name in declared_symbols():
         assert hasattr(oracle.dll, name), name
+
+
+def test_status_codes_match_the_header():
+    """The Python mirror of the status codes and flags equals the enums of include/hfx.h."""
+    text = open(os.path.join(ROOT, "include", "hfx.h")).read()
+    for name, value in re.findall(r"\b(HFX_(?:OK|ERR_\w+|JK_\w+|FP64|MIXED))\s*=\s*(\d+)", text):
+        assert getattr(lib, name) == int(value), name

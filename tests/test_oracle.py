@@ -487,3 +487,13 @@ def test_conventional_scf_matches_direct(oracle):
     assert rc["converged"] == 1 and rd["converged"] == 1
     assert rc["e_total"] == pytest.approx(rd["e_total"], abs=1e-10)
     assert rc["iterations"] == rd["iterations"]
+
+
+def test_basis_errors_are_the_references():
+    """BasisSet failures (src/Basis.cpp:103-109, 161-166): an unknown basis name and an element
+    the basis does not cover are errors, not empty bases."""
+    from hfx import Basis
+    with pytest.raises(ValueError, match="not found"):
+        Basis("no-such-basis", [1], [[0.0, 0.0, 0.0]])
+    with pytest.raises(ValueError, match="not found"):
+        Basis("sto-3g", [92], [[0.0, 0.0, 0.0]])
