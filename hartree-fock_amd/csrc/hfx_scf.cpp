@@ -326,7 +326,10 @@ struct Scf {
     }
 
     // The SCF cycles of SCF::run (SCF.cpp:51-114).
-    int cycles(Diis* diis, bool first_full, int& it_done, bool& conv, hfx_scf_result* res, int trace_base) {
+    // One stage of SCF cycles; with switch_dE > 0 the stage also ends (not converged) once
+    // |dE| < switch_dE after its first iteration.
+    int cycles(Diis* diis, bool first_full, int& it_done, bool& conv, hfx_scf_result* res, int trace_base,
+               double switch_dE = 0.0) {
         conv = false;
         for (iteration = 0; iteration < o.max_iter; ++iteration) {
             const double last = e_elec;
@@ -334,6 +337,10 @@ struct Scf {
             if (rc) return rc;
             if (after_build(diis, last, res, trace_base + iteration)) {
                 conv = true;
+                it_done = iteration + 1;
+                return HFX_OK;
+            }
+            if (switch_dE > 0.0 && iteration > 0 && dE < switch_dE) {
                 it_done = iteration + 1;
                 return HFX_OK;
             }
@@ -448,6 +455,7 @@ void hfx_scf_default_options(hfx_scf_options* o) {
     o->jk_flags = 0;
     o->full_builds = 0;
     o->direct = 1;
+    o->switch_tol = 1.0e-4;
 }
 
 int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz, int32_t charge,
@@ -490,18 +498,22 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
     Diis diis_mixed(s.o.diis_size);
     int its = 0;
     bool conv = false;
-    rc = s.cycles(s.o.use_diis ? &diis_mixed : nullptr, false, its, conv, res, 0);
+    const bool final64 = s.precision == HFX_MIXED && s.o.final_fp64;
+    const bool early = final64 && s.o.switch_tol > 0.0;
+    rc = s.cycles(s.o.use_diis ? &diis_mixed : nullptr, false, its, conv, res, 0, early ? s.o.switch_tol : 0.0);
     if (rc) return rc;
     int total_its = its;
     res->e_total_mixed = s.e_elec + s.e_nuc;
     res->iterations_mixed = its;
-    if (s.precision == HFX_MIXED && s.o.final_fp64) {
+    if (final64) {
         // Restart the incremental sequence from a full FP64 build of the current density and
-        // iterate to convergence again in FP64.
+        // iterate to convergence in FP64: after an early switch the DIIS history of the mixed
+        // builds is kept (their Fock matrices differ from FP64 ones far below the DIIS errors
+        // at that point), otherwise a fresh DIIS starts.
         s.precision = HFX_FP64;
         Diis diis2(s.o.diis_size);
         int its2 = 0;
-        rc = s.cycles(s.o.use_diis ? &diis2 : nullptr, true, its2, conv, res, its);
+        rc = s.cycles(s.o.use_diis ? (early ? &diis_mixed : &diis2) : nullptr, true, its2, conv, res, its);
         if (rc) return rc;
         total_its += its2;
     }
@@ -609,8 +621,11 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
     double t_jk = 0.0;
     int64_t n_builds = 0, n_quart_last = 0;
 
-    auto stage = [&](int precision, bool first_full) -> int {
-        std::vector<Diis> diis(n_mol, Diis(o.diis_size));
+    // One stage over all molecules; a molecule leaves it when converged or, with switch_dE > 0,
+    // once its |dE| < switch_dE.  keep_diis continues the previous stage's DIIS histories.
+    std::vector<Diis> diis;
+    auto stage = [&](int precision, bool first_full, double switch_dE, bool keep_diis) -> int {
+        if (!keep_diis) diis.assign(n_mol, Diis(o.diis_size));
         std::vector<int> active(n_mol, 1);
         for (int m = 0; m < n_mol; ++m) {
             mol[m].precision = precision;
@@ -662,6 +677,9 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
                     active[m] = 0;
                     conv[m] = 1;
                     done[m] = it + 1;
+                } else if (switch_dE > 0.0 && it > 0 && s.dE < switch_dE) {
+                    active[m] = 0;
+                    done[m] = it + 1;
                 }
             }
         }
@@ -674,14 +692,16 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
         return HFX_OK;
     };
 
-    if ((rc = stage(o.precision, false))) return rc;
+    const bool final64 = o.precision == HFX_MIXED && o.final_fp64;
+    const bool early = final64 && o.switch_tol > 0.0;
+    if ((rc = stage(o.precision, false, early ? o.switch_tol : 0.0, false))) return rc;
     for (int m = 0; m < n_mol; ++m) {
         its[m] = done[m];
         its_mixed[m] = done[m];
         e_mixed[m] = mol[m].e_elec + mol[m].e_nuc;
     }
     if (o.precision == HFX_MIXED && o.final_fp64) {
-        if ((rc = stage(HFX_FP64, true))) return rc;
+        if ((rc = stage(HFX_FP64, true, 0.0, early))) return rc;
         for (int m = 0; m < n_mol; ++m) its[m] += done[m];
     }
     const double t_total = secs(t0, clk::now());

@@ -81,7 +81,8 @@ class HfxScfOptions(C.Structure):
                 ("stop_damp_thresh", C.c_double), ("level_shift", C.c_double), ("lshift_gap_tol", C.c_double),
                 ("max_lshift_iter", C.c_int32), ("stop_lshift_thresh", C.c_double), ("precision", C.c_int32),
                 ("fp32_cut", C.c_double), ("final_fp64", C.c_int32), ("jk_flags", C.c_int32),
-                ("full_builds", C.c_int32), ("direct", C.c_int32)]
+                ("full_builds", C.c_int32), ("direct", C.c_int32),
+                ("switch_tol", C.c_double)]
 
 
 class HfxScfResult(C.Structure):

@@ -46,6 +46,10 @@ typedef struct hfx_scf_options {
     int32_t direct;            /* DIRECT             1: direct builds; 0: conventional, the  */
                                /* integrals stored once (hfx_store_eris) and every build from */
                                /* the full density through them (FP64; src/SCF.cpp:57-60)     */
+    double switch_tol;         /* MIXED + final_fp64: leave the mixed builds once |dE| falls  */
+                               /* below this, then continue with FP64 builds (the first one   */
+                               /* full) keeping the DIIS history; 0: converge in mixed        */
+                               /* precision first, then restart FP64 with a fresh DIIS        */
 } hfx_scf_options;
 
 typedef struct hfx_scf_result {

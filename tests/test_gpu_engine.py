@@ -195,11 +195,15 @@ def test_scf_tight_parity(engine_lib, oracle):
     assert np.abs(rg["F_alpha"] - ro["F_alpha"]).max() <= 1e-8
 
 
-def test_scf_mixed_recovers_fp64(engine_lib):
+@pytest.mark.parametrize("switch_tol", [1e-4, 0.0])
+def test_scf_mixed_recovers_fp64(engine_lib, switch_tol):
+    """Mixed builds, then FP64 from a full build: switching early (DIIS history kept) or after
+    converging in mixed precision (fresh DIIS) both end within 1e-6 Eh of the FP64 SCF."""
     b = molecules.ethane()
     kw = dict(energy_tol=1e-9)
     r64 = lib.scf_run(engine_lib, b, options=engine_lib.default_options(**kw))
     rmx = lib.scf_run(engine_lib, b, options=engine_lib.default_options(precision=lib.HFX_MIXED, fp32_cut=1e-4,
-                                                                         final_fp64=1, **kw))
+                                                                         final_fp64=1, switch_tol=switch_tol, **kw))
     assert r64["converged"] == 1 and rmx["converged"] == 1
     assert abs(rmx["e_total"] - r64["e_total"]) <= 1e-6
+    assert 0 < rmx["iterations_mixed"] < rmx["iterations"]

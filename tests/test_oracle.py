@@ -497,3 +497,26 @@ def test_basis_errors_are_the_references():
         Basis("no-such-basis", [1], [[0.0, 0.0, 0.0]])
     with pytest.raises(ValueError, match="not found"):
         Basis("sto-3g", [92], [[0.0, 0.0, 0.0]])
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_mixed_precision_switch_control_flow(oracle, batched):
+    """The driver's precision stages (the oracle builds everything in FP64, so only the control
+    flow differs): with switch_tol the first stage ends once |dE| < switch_tol and the FP64 stage
+    keeps the DIIS history, reaching the FP64 energy in as many iterations as the FP64 SCF (+1);
+    with switch_tol = 0 the first stage converges and the FP64 stage restarts."""
+    mols = [molecules.water_dimer("6-31g"), molecules.h2o("6-31g**")]
+    kw = dict(energy_tol=1e-8, schwarz_thresh=1e-10, density_thresh=1e-10, max_iter=60)
+    run = (lambda o: lib.scf_run_batched(oracle, mols, options=o)) if batched else \
+        (lambda o: [lib.scf_run(oracle, mols[0], options=o)])
+    r64 = run(oracle.default_options(**kw))
+    for sw in (1e-4, 0.0):
+        rm = run(oracle.default_options(precision=lib.HFX_MIXED, fp32_cut=1e-2, final_fp64=1, switch_tol=sw, **kw))
+        for a, b in zip(rm, r64):
+            assert a["converged"] == 1
+            assert a["e_total"] == pytest.approx(b["e_total"], abs=1e-7)  # both converged to dE < 1e-8
+            if sw > 0:
+                assert a["iterations_mixed"] < b["iterations"] - 2
+                assert a["iterations"] <= b["iterations"] + 1
+            else:
+                assert a["iterations_mixed"] == b["iterations"]
