@@ -31,7 +31,8 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-I" + INC,
 CXXFLAGS = ["-O3", "-std=c++20", "-fPIC", "-fopenmp", "-I" + INC, "-Wall"]
 
 ACC_MAX = 36          # must match k_eri.hip
-CHUNKS_PER_TU = 6     # chunks per translation unit (one launch each; x2 for FP64/FP32)
+CHUNKS_PER_TU = 6     # chunks per translation unit (one launch each; FP64, FP32 and stored-digest kernels)
+CHUNKS_PER_TU_HEAVY = 3  # the same for the classes with la+lb+lc+ld >= 6, whose units compile longest
 
 
 def ncart(l):
@@ -67,8 +68,9 @@ def jobs():
     srcs.append(("engine", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "engine.hip")]))
     srcs.append(("k_grad", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "k_grad.hip")]))
     for (k, la, lb, lc, ld, nch) in combo_classes():
-        for b in range(0, nch, CHUNKS_PER_TU):
-            e = min(nch, b + CHUNKS_PER_TU)
+        step = CHUNKS_PER_TU_HEAVY if la + lb + lc + ld >= 6 else CHUNKS_PER_TU
+        for b in range(0, nch, step):
+            e = min(nch, b + step)
             name = "k_eri_%02d_%02d" % (k, b)
             srcs.append((name, [HIPCC] + HIPFLAGS + ["-DHFX_COMBO=%d" % k, "-DHFX_CHUNK_BEGIN=%d" % b,
                                                      "-DHFX_CHUNK_END=%d" % e, "-c",
