@@ -72,8 +72,17 @@ class Shell:
         return (self.l + 1) * (self.l + 2) // 2
 
 
-def read_basis(name, elements):
-    """Raw shells per element: ``{Z: [(l, exponents, coefficients), ...]}``."""
+def read_basis(name, elements, general="reference"):
+    """Raw shells per element: ``{Z: [(l, exponents, coefficients), ...]}``.
+
+    general = "reference": the reference's reading (src/Basis.cpp:125-159): an entry with angular
+    momenta [l_0, l_1, ...] gives one shell per l_i with coefficient row i, so the extra rows of
+    a general contraction (cc-pVDZ's s and p entries) are not read.
+    general = "expand": every coefficient row of an entry becomes a shell of its own (with the
+    entry's angular momentum for a single-l entry, l_i for row i of an sp-style entry), and the
+    primitives whose coefficient in that row is zero are dropped."""
+    if general not in ("reference", "expand"):
+        raise ValueError("general must be 'reference' or 'expand'")
     path = os.path.join(DATA_DIR, basis_file_name(name) + ".json")
     if not os.path.exists(path):
         raise ValueError("Basis set '%s' not found." % name)
@@ -87,10 +96,16 @@ def read_basis(name, elements):
         raw = []
         for sh in data["elements"][key]["electron_shells"]:
             am = sh["angular_momentum"]
-            for i, l in enumerate(am):
-                exps = [float(s) for s in sh["exponents"]]
-                coefs = [float(s) for s in sh["coefficients"][i]]
-                raw.append((int(l), exps, coefs))
+            exps = [float(s) for s in sh["exponents"]]
+            if general == "reference" or len(sh["coefficients"]) <= len(am):
+                for i, l in enumerate(am):
+                    coefs = [float(s) for s in sh["coefficients"][i]]
+                    raw.append((int(l), exps, coefs))
+            else:
+                for i, row in enumerate(sh["coefficients"]):
+                    l = int(am[i] if len(am) > 1 else am[0])
+                    keep = [k for k, c in enumerate(row) if float(c) != 0.0]
+                    raw.append((l, [exps[k] for k in keep], [float(row[k]) for k in keep]))
         out[z] = raw
     return out
 
@@ -110,11 +125,11 @@ def _norm_factors(l, exponents):
 class Basis:
     """A basis on a molecule: an ordered list of shells plus the flat engine layout."""
 
-    def __init__(self, name, Z, xyz):
+    def __init__(self, name, Z, xyz, general="reference"):
         self.name = name
         self.Z = [int(z) for z in Z]
         self.xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
-        raw = read_basis(name, sorted(set(self.Z)))
+        raw = read_basis(name, sorted(set(self.Z)), general)
         self.shells = []
         for ia, z in enumerate(self.Z):
             for (l, exps, coefs) in raw[z]:

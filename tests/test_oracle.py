@@ -520,3 +520,19 @@ def test_mixed_precision_switch_control_flow(oracle, batched):
                 assert a["iterations"] <= b["iterations"] + 1
             else:
                 assert a["iterations_mixed"] == b["iterations"]
+
+
+def test_general_contraction_expansion():
+    """Reading cc-pVDZ with every contraction row as a shell (zero-coefficient primitives dropped)
+    gives exactly the shells of the committed cc-pvdz-expanded data, while the reference's reading
+    (one row per angular momentum, src/Basis.cpp:125-159) keeps only the first row."""
+    from hfx import Basis
+    Z = [1, 6, 7, 8]
+    xyz = np.arange(12, dtype=float).reshape(4, 3)
+    a = Basis("cc-pvdz", Z, xyz, general="expand")
+    b = Basis("cc-pvdz-expanded", Z, xyz)
+    assert [(s.l, s.exponents, s.coefficients, s.atom) for s in a.shells] == \
+        [(s.l, s.exponents, s.coefficients, s.atom) for s in b.shells]
+    r = Basis("cc-pvdz", Z, xyz)
+    assert r.n_aos < a.n_aos
+    assert Basis("sto-3g", Z, xyz, general="expand").n_aos == Basis("sto-3g", Z, xyz).n_aos
