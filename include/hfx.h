@@ -40,7 +40,8 @@ enum {
     HFX_JK_USE_GRAPH = 2,   /* replay the engine's captured hipGraph of the build pipeline   */
     HFX_JK_NO_STATS = 4,    /* skip the survivor statistics read-back                         */
     HFX_JK_CLASS_TIMING = 8, /* time every class kernel group with HIP events (stats.class_ms*) */
-    HFX_JK_STORED = 16       /* digest the integrals kept by hfx_store_eris: no density test, FP64 only;
+    HFX_JK_STORED = 16       /* digest the integrals kept by hfx_store_eris: no density test, FP64 only,
+                              * args->Q and density_thresh unused (the stored list is fixed);
                               * the conventional build SCF::buildFockMatrix (src/SCF.cpp:233-521) */
 };
 
