@@ -724,7 +724,8 @@ struct hfx_engine {
     int screen_nbra[N_COMBOS] = {};          // their bra counts (0: class without work)
     unsigned int* d_chunk_ctr = nullptr;     // [2*N_COMBOS*CHUNK_SLOTS]: chunk counters per class kernel
     static constexpr int CHUNK_SLOTS = 64;   // chunk kernels per class, at most
-    static constexpr int CHUNK_LEN = 2048;   // list entries a block claims at a time
+    static constexpr int CHUNK_LEN = 2048;   // list entries a block claims at a time (HFX_CHUNK_LEN to change)
+    static constexpr int STORED_CHUNK_LEN = 8192;  // the same for stored builds
     int max_N = 0;
 
     // stored integrals (hfx_store_eris): the Schwarz-screened list of every class, its ranges and
@@ -1424,7 +1425,13 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     if (P.stored) ea.list = e->d_slist;
     const int grid = eri_grid(e);
     hipLaunchKernelGGL(k_fill_u32, dim3(4), dim3(256), 0, s, e->d_chunk_ctr, 2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
-    ea.chunk_len = hfx_engine::CHUNK_LEN;
+    // Longer runs flush a stored build's exchange rows less often; in the direct build they leave
+    // a long tail on the heaviest-first lists (ssss FP32 2.9 -> 7.5 ms alone at 8192).
+    static const int chunk_env = [] {
+        const char* v = std::getenv("HFX_CHUNK_LEN");
+        return v ? std::max(256, std::min(1 << 20, std::atoi(v))) : 0;
+    }();
+    ea.chunk_len = chunk_env ? chunk_env : (P.stored ? hfx_engine::STORED_CHUNK_LEN : hfx_engine::CHUNK_LEN);
     // LDS exchange rows of a class kernel: NA + NB rows of max_N doubles (twice for UHF), next
     // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without.
     auto tile_cols = [&](int k, int fp32) {
