@@ -129,3 +129,17 @@ def test_stored_water8_against_direct(engine_lib):
     scale = max(1.0, np.abs(Jd).max())
     assert np.abs(Js - Jd).max() <= 1e-12 * scale
     assert np.abs(Ks - Kd).max() <= 1e-12 * scale
+
+
+def test_conventional_batched_scf_matches_direct(engine_lib, oracle):
+    """DIRECT false in the batched driver: one store for all molecules, full-density builds through
+    it; per-molecule energies equal the direct full-build batched run and the oracle's."""
+    mols = [molecules.h2o("6-31g**"), molecules.water_dimer("6-31g"), molecules.methane()]
+    kw = dict(energy_tol=1e-8, max_iter=60, schwarz_thresh=1e-12, density_thresh=1e-12)
+    rc = lib.scf_run_batched(engine_lib, mols, options=engine_lib.default_options(direct=0, **kw))
+    rd = lib.scf_run_batched(engine_lib, mols, options=engine_lib.default_options(direct=1, full_builds=1, **kw))
+    ro = lib.scf_run_batched(oracle, mols, options=oracle.default_options(direct=0, **kw))
+    for a, b, c in zip(rc, rd, ro):
+        assert a["converged"] == 1 and b["converged"] == 1 and c["converged"] == 1
+        assert a["e_total"] == pytest.approx(b["e_total"], abs=1e-9)
+        assert a["e_total"] == pytest.approx(c["e_total"], abs=1e-9)

@@ -536,3 +536,17 @@ def test_general_contraction_expansion():
     r = Basis("cc-pvdz", Z, xyz)
     assert r.n_aos < a.n_aos
     assert Basis("sto-3g", Z, xyz, general="expand").n_aos == Basis("sto-3g", Z, xyz).n_aos
+
+
+def test_conventional_batched_scf(oracle):
+    """The batched driver's conventional mode (stored tensor per molecule) reaches the same
+    per-molecule energies as its direct full-build mode and as single-molecule runs."""
+    mols = [molecules.h2o("6-31g"), molecules.water_dimer("6-31g")]
+    kw = dict(energy_tol=1e-8, max_iter=60, schwarz_thresh=1e-12, density_thresh=1e-12)
+    rc = lib.scf_run_batched(oracle, mols, options=oracle.default_options(direct=0, **kw))
+    rd = lib.scf_run_batched(oracle, mols, options=oracle.default_options(direct=1, full_builds=1, **kw))
+    for m, a, b in zip(mols, rc, rd):
+        rs = lib.scf_run(oracle, m, options=oracle.default_options(direct=0, **kw))
+        assert a["converged"] == 1 and b["converged"] == 1
+        assert a["e_total"] == pytest.approx(b["e_total"], abs=1e-9)
+        assert a["e_total"] == pytest.approx(rs["e_total"], abs=1e-9)
