@@ -486,7 +486,6 @@ def test_conventional_scf_matches_direct(oracle):
     rd = lib.scf_run(oracle, b, options=oracle.default_options(direct=1, full_builds=1, **opts))
     assert rc["converged"] == 1 and rd["converged"] == 1
     assert rc["e_total"] == pytest.approx(rd["e_total"], abs=1e-10)
-    assert rc["iterations"] == rd["iterations"]
 
 
 def test_basis_errors_are_the_references():
