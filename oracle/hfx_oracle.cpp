@@ -500,7 +500,11 @@ void kinetic_pair(const Mol& m, int sa, int sb, double* T) {
                     if (l2 >= 2) t3 -= l2 * (l2 - 1) * Ex.at(l1, l2 - 2, 0) * sy * sz;
                     if (m2 >= 2) t3 -= m2 * (m2 - 1) * sx * Ey.at(m1, m2 - 2, 0) * sz;
                     if (n2 >= 2) t3 -= n2 * (n2 - 1) * sx * sy * Ez.at(n1, n2 - 2, 0);
+#ifdef ORACLE_PLANT_KINETIC_FACTOR
+                    // planted mistake for the mutation test: l(l-1) instead of l(l-1)/2
+#else
                     t3 *= 0.5;
+#endif
                     T[i + (size_t)j * N] += na * nb * cp * pref * (t1 + t2 + t3);
                 }
             }
@@ -541,7 +545,12 @@ void nuclear_pair(const Mol& m, int nat, const double* Z, const double* xyz, int
                             double su = 0.0;
                             for (int u = 0; u <= m1 + m2; ++u) {
                                 double st = 0.0;
+#ifdef ORACLE_PLANT_NUCLEAR_TRANSPOSE
+                                // planted mistake for the mutation test: u and v of R transposed
+                                for (int t = 0; t <= l1 + l2; ++t) st += Ex.at(l1, l2, t) * R.at(t, v, u, 0);
+#else
                                 for (int t = 0; t <= l1 + l2; ++t) st += Ex.at(l1, l2, t) * R.at(t, u, v, 0);
+#endif
                                 su += Ey.at(m1, m2, u) * st;
                             }
                             sum += Ez.at(n1, n2, v) * su;
@@ -1327,6 +1336,19 @@ static int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st, 0, 1 << 30);
+}
+
+// The CPU oracle has no device and no stream: the "device" buffers of the async variant are host
+// buffers and the build completes before the call returns (HFX_JK_DETERMINISTIC is accepted and
+// changes nothing here).
+int hfx_jk_async(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, void* stream) {
+    (void)stream;
+    if (!args) return fail(HFX_ERR_INVALID, "hfx_jk_async: null argument");
+    if (!(args->flags & HFX_JK_DEVICE_PTRS) || (args->flags & HFX_JK_CLASS_TIMING))
+        return fail(HFX_ERR_INVALID, "hfx_jk_async: needs HFX_JK_DEVICE_PTRS and no HFX_JK_CLASS_TIMING");
+    hfx_jk_args a = *args;
+    a.flags &= ~HFX_JK_DEVICE_PTRS;
+    return jk_impl(e, &a, J, Ka, Kb, nullptr, 0, 1 << 30);
 }
 
 int hfx_store_eris(hfx_engine* e, const double* Q, double schwarz_thresh, int64_t* n_quartets, int64_t* n_bytes) {

@@ -12,16 +12,18 @@ The reference reader keeps only ``coefficients[i]`` for ``i < len(angular_moment
 (``src/Basis.cpp:125-127,145-146``), so a general contraction written as one entry loses
 its extra rows; written one row per entry, all rows survive.
 
-Run once, in a tree where ``/root/reference`` exists::
+This is the provenance script of the committed data in ``hartree-fock_amd/hfx/basis_data/``;
+the package never runs it.  Run once, in a tree where ``/root/reference`` exists::
 
-    python hartree-fock_amd/hfx/basis_data/extract.py
+    python tests/golden/extract_basis_data.py
 """
 import json
 import os
 import sys
 
 REF = "/root/reference/basis_sets"
-HERE = os.path.dirname(os.path.abspath(__file__))
+HERE = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                    "hartree-fock_amd", "hfx", "basis_data")
 ELEMENTS = ["1", "6", "7", "8"]
 FILES = {
     "sto-3g": "sto-3g.json",

@@ -44,6 +44,20 @@ def test_one_electron(pairs, mol):
     assert np.abs(x - y).max() <= 1e-11 * max(1.0, np.abs(y).max())
 
 
+def test_one_electron_against_obara_saika(pairs):
+    """GPU S, T, V of every shell pair of methane / cc-pVDZ ((la, lb) through (d, d)) against the
+    independent Obara-Saika recurrences of tests/os_reference.py, not only against the oracle."""
+    b, g, _ = pairs["methane_ccpvdz"]
+    S, T, V = g.overlap(), g.kinetic(), g.nuclear(b.Z, b.xyz)
+    off = b.flat()["ao_offset"]
+    for sa, A in enumerate(b.shells):
+        for sb, B in enumerate(b.shells[: sa + 1]):
+            ref = os_reference.one_electron_block(b, sa, sb, b.Z, b.xyz)
+            for M, want in zip((S, T, V), ref):
+                got = M[off[sa]:off[sa] + A.nao, off[sb]:off[sb] + B.nao]
+                assert np.abs(got - want).max() <= 1e-11 * max(1.0, np.abs(want).max()), (sa, sb)
+
+
 @pytest.mark.parametrize("mol", list(MOLS))
 def test_schwarz(pairs, mol):
     b, g, o = pairs[mol]

@@ -3,6 +3,7 @@ J/K loop) against mathematics that does not depend on it: mpmath Boys values, cl
 of E and R, an independent Obara-Saika ERI evaluation, textbook values and brute force."""
 import ctypes as C
 import math
+import os
 
 import mpmath
 import numpy as np
@@ -143,6 +144,68 @@ def test_overlap_normalization(oracle):
                 # <x^2|x^2> for d_xx has norm 3/3 = 1 with the (2i-1)!! factors
                 assert S[i, i] == pytest.approx(1.0, rel=1e-13)
     assert np.allclose(S, S.T, atol=1e-15)
+
+
+@pytest.mark.parametrize("mol", ["methane_ccpvdz", "h2o_631gss"])
+def test_one_electron_against_obara_saika(oracle, mol):
+    """S, T and V of every shell pair, (la, lb) through (d, d), against the independent Obara-Saika
+    recurrences (reference IntegralEngine.cpp:78-142 overlap, 162-238 kinetic with its l(l-1) term
+    for l >= 2 at 226-232, 259-352 nuclear attraction)."""
+    b = molecules.methane() if mol == "methane_ccpvdz" else molecules.h2o("6-31g**")
+    e = lib.Engine(oracle, b)
+    S, T, V = e.overlap(), e.kinetic(), e.nuclear(b.Z, b.xyz)
+    off = b.flat()["ao_offset"]
+    seen = set()
+    for sa, A in enumerate(b.shells):
+        for sb, B in enumerate(b.shells[: sa + 1]):
+            ref = os_reference.one_electron_block(b, sa, sb, b.Z, b.xyz)
+            blk = [M[off[sa]:off[sa] + A.nao, off[sb]:off[sb] + B.nao] for M in (S, T, V)]
+            for name, got, want in zip("STV", blk, ref):
+                scale = max(1.0, np.abs(want).max())
+                assert np.abs(got - want).max() <= 1e-12 * scale, (name, sa, sb, A.l, B.l)
+            seen.add((max(A.l, B.l), min(A.l, B.l)))
+    assert seen == {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)}
+
+
+def _one_electron_mismatch(L, b):
+    """Largest scaled |oracle - Obara-Saika| over the shell pairs of b, per (la, lb) class."""
+    e = lib.Engine(L, b)
+    S, T, V = e.overlap(), e.kinetic(), e.nuclear(b.Z, b.xyz)
+    off = b.flat()["ao_offset"]
+    worst = {}
+    for sa, A in enumerate(b.shells):
+        for sb, B in enumerate(b.shells[: sa + 1]):
+            ref = os_reference.one_electron_block(b, sa, sb, b.Z, b.xyz)
+            blk = [M[off[sa]:off[sa] + A.nao, off[sb]:off[sb] + B.nao] for M in (S, T, V)]
+            for name, got, want in zip("STV", blk, ref):
+                k = (name, max(A.l, B.l), min(A.l, B.l))
+                worst[k] = max(worst.get(k, 0.0), np.abs(got - want).max() / max(1.0, np.abs(want).max()))
+    return worst
+
+
+@pytest.mark.parametrize("plant", ["ORACLE_PLANT_KINETIC_FACTOR", "ORACLE_PLANT_NUCLEAR_TRANSPOSE"])
+def test_obara_saika_check_catches_planted_oracle_mistakes(tmp_path, plant):
+    """Mutation test of the check above: the oracle built with a planted slip (the kinetic
+    l(l-1) term without its factor 1/2, IntegralEngine.cpp:226-232; or the u and v indices of
+    R^0_tuv transposed in the nuclear attraction, :325-340) fails the Obara-Saika comparison on
+    exactly the classes the slip reaches, and still passes the others."""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    odir = os.path.join(os.path.dirname(here), "oracle")
+    so = str(tmp_path / "liboracle_planted.so")
+    subprocess.check_call(["g++", "-std=c++20", "-fPIC", "-shared", "-fopenmp", "-O1", "-ffp-contract=off",
+                           "-I" + os.path.join(os.path.dirname(here), "include"), "-D" + plant,
+                           os.path.join(odir, "hfx_oracle.cpp"),
+                           os.path.join(os.path.dirname(here), "hartree-fock_amd", "csrc", "hfx_scf.cpp"),
+                           "-o", so])
+    worst = _one_electron_mismatch(lib.Library(so), molecules.h2o("6-31g**"))
+    if plant == "ORACLE_PLANT_KINETIC_FACTOR":
+        hit = {k for k, v in worst.items() if v > 1e-9}
+        assert hit and all(k[0] == "T" and k[1] == 2 for k in hit), hit   # only T of pairs with a d shell
+    else:
+        hit = {k for k, v in worst.items() if v > 1e-9}
+        assert hit and all(k[0] == "V" and k[1] >= 1 for k in hit), hit   # only V, never ss
+    assert all(v <= 1e-12 for k, v in worst.items() if k not in hit)
 
 
 def test_ssss_closed_form(oracle):
