@@ -6,6 +6,7 @@ Translation units:
   csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
                               are fully unrolled, so they are compiled in parallel slices
   csrc/hfx_scf.cpp            host SCF driver (g++ with OpenMP)
+  tests/c_caller/hfx_c_caller a C++ test program calling the library through include/hfx.h (g++)
 
 Usage: python build.py [-j N] [--only engine hfx_scf ...]
   --only recompiles just the named objects and links them with the existing others.
@@ -124,9 +125,27 @@ def build(jobs_n=None, verbose=False, only=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(LIB + ".tmp", LIB)
+    build_c_caller()
     if verbose:
         print("built %s in %.1fs (%d objects)" % (LIB, time.time() - t0, len(objs)), flush=True)
     return LIB
+
+
+C_CALLER_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_caller", "hfx_c_caller.cpp")
+C_CALLER_BIN = os.path.join(os.path.dirname(HERE), "tests", "c_caller", "hfx_c_caller")
+
+
+def build_c_caller():
+    """The test program that calls the library through include/hfx.h from C++ (g++, no HIP headers)."""
+    if not os.path.exists(C_CALLER_SRC):
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + INC, C_CALLER_SRC, "-o", C_CALLER_BIN + ".tmp",
+           "-L" + LIB_DIR, "-lhfx_mi355x", "-Wl,-rpath,$ORIGIN/../../hartree-fock_amd/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C caller build failed:\n" + r.stderr[-4000:])
+    os.replace(C_CALLER_BIN + ".tmp", C_CALLER_BIN)
+    return C_CALLER_BIN
 
 
 if __name__ == "__main__":

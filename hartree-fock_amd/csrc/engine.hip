@@ -216,6 +216,7 @@ struct ScreenArgs {
     const ScreenClass* cls;  // [N_COMBOS]
     // per pair / shell data
     const int4* pair_info;  // {shell a, shell b, local a | local b << 16, nprim pairs}
+    const int* pair_npx;    // primitive pairs the kernels execute: FP64 count | FP32 count << 16
     const int* pair_mol;
     const double* pair_q;
     const long long* sh_qbase;
@@ -229,7 +230,8 @@ struct ScreenArgs {
     uint2* list;
     unsigned long long* counts;        // pass 0: survivors of bra i at [slot64 + i] / [slot32 + i]
     const unsigned long long* offs;    // pass 1: exclusive scan of counts = list positions
-    unsigned long long* stats;  // [4 k]: hash sum, hash xor, primitive quartets FP64 / FP32 of class k
+    unsigned long long* stats;  // [6 k]: hash sum, hash xor, primitive quartets FP64 / FP32 of class k
+                                // (unpruned model counts), then the executed ones FP64 / FP32
 };
 
 // One wavefront per bra pair (a, b).  Ket candidates: for every nprim segment of the bra's
@@ -264,6 +266,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     long long qb = 0, ns = 0;
     double qab = 0.0, tab2 = 0.0;
     long long npa = 0;
+    int npxa = 0;
     if (active) {
         pid = K.bra_ids[bpos];
         const int4 pi = a.pair_info[pid];
@@ -272,6 +275,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
         la = pi.z & 0xffff;
         lb = pi.z >> 16;
         npa = pi.w;
+        npxa = a.pair_npx[pid];
         mol = a.pair_mol[pid];
         qab = a.pair_q[pid];
         qb = a.sh_qbase[sa];
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
     const double* ra = rows;
     const double* rb = rows + a.ns_max;
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0;
+    unsigned long long h_sum = 0, h_xor = 0, prims = 0, prims32 = 0, ex64 = 0, ex32 = 0;
     const int sg0 = K.ket_molseg[mol], sg1 = K.ket_molseg[mol + 1];
     unsigned long long n64 = 0, n32 = 0, b64 = 0, b32 = 0;
     if (PASS >= 1) {
@@ -347,10 +351,13 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
                         a.list[(long long)(b64 + __popcll(m64 & lt_mask))] = ent;
                     else
                         a.list[(long long)(b32 + __popcll(m32 & lt_mask))] = ent;
-                    if (f32)
+                    if (f32) {
                         prims32 += (unsigned long long)(npa * pk.z);
-                    else
+                        ex32 += (unsigned long long)((npxa >> 16) * (pk.w >> 16));
+                    } else {
                         prims += (unsigned long long)(npa * pk.z);
+                        ex64 += (unsigned long long)((npxa & 0xffff) * (pk.w & 0xffff));
+                    }
                     if constexpr (PASS == 1) {
                         const int4 pq = a.pair_info[qid];
                         const int sc = pq.x, sd = pq.y;
@@ -387,12 +394,16 @@ __global__ __launch_bounds__(256) void k_screen(ScreenArgs a) {
         h_xor ^= __shfl_xor(h_xor, off);
         prims += __shfl_xor(prims, off);
         prims32 += __shfl_xor(prims32, off);
+        ex64 += __shfl_xor(ex64, off);
+        ex32 += __shfl_xor(ex32, off);
     }
     if (lane == 0) {
-        if (h_sum) atomicAdd(&a.stats[4 * k + 0], h_sum);
-        if (h_xor) atomicXor(&a.stats[4 * k + 1], h_xor);
-        if (prims) atomicAdd(&a.stats[4 * k + 2], prims);
-        if (prims32) atomicAdd(&a.stats[4 * k + 3], prims32);
+        if (h_sum) atomicAdd(&a.stats[6 * k + 0], h_sum);
+        if (h_xor) atomicXor(&a.stats[6 * k + 1], h_xor);
+        if (prims) atomicAdd(&a.stats[6 * k + 2], prims);
+        if (prims32) atomicAdd(&a.stats[6 * k + 3], prims32);
+        if (ex64) atomicAdd(&a.stats[6 * k + 4], ex64);
+        if (ex32) atomicAdd(&a.stats[6 * k + 5], ex32);
     }
 }
 
@@ -422,6 +433,29 @@ __global__ void k_symm(const double* A, long long stride, int copies, double* ou
         const long long t = tb + tri_index(r, c);
         double v = 0.0;
         for (int k = 0; k < copies; ++k) v += A[k * stride + t];
+        out[base + e] = (r == c) ? 2.0 * v : v;
+    }
+}
+
+// The same for fixed-point accumulators (element t of a copy is the int64 pair [2t, 2t+1] =
+// (hi, lo), value hi 2^-32 + lo 2^-72): the copies are summed as integers, so the result does
+// not depend on how the additions were distributed over the copies.
+__global__ void k_symm_fixed(const double* A, long long stride, int copies, double* out, const long long* mbase,
+                             const long long* tbase, const int* mn) {
+    const int m = blockIdx.y;
+    const long long base = mbase[m], tb = tbase[m];
+    const long long N = mn[m];
+    const long long n = N * N;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e % N, c = e / N;
+        const long long t = tb + tri_index(r, c);
+        unsigned long long hi = 0, lo = 0;
+        for (int k = 0; k < copies; ++k) {
+            const unsigned long long* q = reinterpret_cast<const unsigned long long*>(A + k * stride) + 2 * t;
+            hi += q[0];
+            lo += q[1];
+        }
+        const double v = ldexp((double)(long long)hi, -32) + ldexp((double)(long long)lo, -72);
         out[base + e] = (r == c) ? 2.0 * v : v;
     }
 }
@@ -598,9 +632,10 @@ struct BuildParams {
     bool mixed, uhf, timing;
     bool hash = true;     // the survivor-list hash (statistics only)
     bool stored = false;  // digest the stored integrals (no screening pass)
+    bool det = false;     // fixed-point accumulators (bitwise-reproducible J/K)
     bool operator==(const BuildParams& o) const {
         return tauS == o.tauS && tauD == o.tauD && cut == o.cut && mixed == o.mixed && uhf == o.uhf &&
-               timing == o.timing && hash == o.hash && stored == o.stored;
+               timing == o.timing && hash == o.hash && stored == o.stored && det == o.det;
     }
 };
 
@@ -635,6 +670,7 @@ struct hfx_engine {
     hipStream_t side[N_SIDE] = {};
     hipEvent_t ev_fork = nullptr, ev_join[N_SIDE] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_async = nullptr;  // orders hfx_jk_async builds against the engine's stream
     DevArena mem;
 
     // molecules
@@ -671,6 +707,7 @@ struct hfx_engine {
     std::vector<int> pair_ps32, pair_np32;
     long long n_pp32 = 0;
     int4* d_pair_info = nullptr;
+    int* d_pair_npx = nullptr;  // pair_np | pair_np32 << 16
     int ns_max = 0;
     double* d_pair_q = nullptr;
     PrimPair64* d_pp64 = nullptr;
@@ -714,7 +751,7 @@ struct hfx_engine {
     long long* d_slots = nullptr;
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
-    unsigned long long* d_stats = nullptr;  // [4 * N_COMBOS]
+    unsigned long long* d_stats = nullptr;  // [6 * N_COMBOS]
     hipEvent_t cev[2 * N_COMBOS + 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -763,6 +800,7 @@ struct hfx_engine {
             if (x) (void)hipEventDestroy(x);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_async) (void)hipEventDestroy(ev_async);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         for (int i = 1; i < N_SIDE; ++i) {
             if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
@@ -846,11 +884,13 @@ int sort_pairs(hfx_engine* e, const std::vector<double>& qt) {
         int rc;
         if ((rc = put(e->d_cls_ids[pc], ids)) || (rc = put(e->d_cls_q[pc], e->cls_q[pc]))) return rc;
         {
-            // what the screening needs of each position, in one load: {pair, local a | local b << 16, nprim, 0}
+            // what the screening needs of each position, in one load:
+            // {pair, local a | local b << 16, unpruned nprim, executed FP64 | FP32 nprim << 16}
             std::vector<int4> info(ids.size());
             for (size_t i = 0; i < ids.size(); ++i) {
                 const int p = ids[i];
-                info[i] = make_int4(p, e->sh_local[e->pair_a[p]] | (e->sh_local[e->pair_b[p]] << 16), e->pair_nprim[p], 0);
+                info[i] = make_int4(p, e->sh_local[e->pair_a[p]] | (e->sh_local[e->pair_b[p]] << 16), e->pair_nprim[p],
+                                    e->pair_np[p] | (e->pair_np32[p] << 16));
             }
             if ((rc = put(e->d_cls_info[pc], info))) return rc;
         }
@@ -1035,6 +1075,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     if (const char* v = std::getenv("HFX_BUILD_STREAMS")) e->n_side = std::max(1, std::min(hfx_engine::N_SIDE, std::atoi(v)));
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
+    HIPCHK(hipEventCreateWithFlags(&e->ev_async, hipEventDisableTiming));
     for (hipEvent_t& x : e->cev) HIPCHK(hipEventCreate(&x));
 
     // ---- flatten the molecules' bases ----
@@ -1175,6 +1216,12 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
         const int a = e->pair_a[p], b = e->pair_b[p];
         pinfo[p] = make_int4(a, b, e->sh_local[a] | (e->sh_local[b] << 16), e->pair_nprim[p]);
     }
+    std::vector<int> npx(e->n_pairs);
+    for (int p = 0; p < e->n_pairs; ++p) {
+        if (e->pair_np[p] > 65535 || e->pair_np32[p] > 32767)
+            return fail(HFX_ERR_UNSUPPORTED, "more than 32767 primitive pairs in one shell pair");
+        npx[p] = e->pair_np[p] | (e->pair_np32[p] << 16);
+    }
     for (int m = 0; m < n_mol; ++m) e->ns_max = std::max(e->ns_max, e->mol_nS[m]);
     if (e->ns_max > 65535) return fail(HFX_ERR_UNSUPPORTED, "more than 65535 shells in one molecule");
     // ---- Boys tables ----
@@ -1231,7 +1278,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
         (rc = upload(e.get(), e->d_boys32, b32)) || (rc = upload(e.get(), e->d_mbase, e->mol_mbase)) ||
         (rc = upload(e.get(), e->d_mn, e->mol_N)) || (rc = upload(e.get(), e->d_reg, e->reg)) ||
         (rc = upload(e.get(), e->d_pab64, pab64)) || (rc = upload(e.get(), e->d_pab32, pab32)) ||
-        (rc = upload(e.get(), e->d_sh_l, e->sh_l)))
+        (rc = upload(e.get(), e->d_sh_l, e->sh_l)) || (rc = upload(e.get(), e->d_pair_npx, npx)))
         return rc;
     const long long ne = e->n_elems, nq = e->n_qelems;
     e->d_Dt = e->mem.alloc<double>(ne);
@@ -1244,9 +1291,10 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     e->d_Tt = e->mem.alloc<double>(nt);
     e->d_Ta = e->mem.alloc<double>(nt);
     e->d_Tb = e->mem.alloc<double>(nt);
-    e->d_AJ = e->mem.alloc<double>(nt * hfx_engine::ACC_COPIES);
-    e->d_AKa = e->mem.alloc<double>(nt * hfx_engine::ACC_COPIES);
-    e->d_AKb = e->mem.alloc<double>(nt * hfx_engine::ACC_COPIES);
+    // two words per element, for the fixed-point pairs of deterministic builds
+    e->d_AJ = e->mem.alloc<double>(2 * nt * hfx_engine::ACC_COPIES);
+    e->d_AKa = e->mem.alloc<double>(2 * nt * hfx_engine::ACC_COPIES);
+    e->d_AKb = e->mem.alloc<double>(2 * nt * hfx_engine::ACC_COPIES);
     e->d_J = e->mem.alloc<double>(ne);
     e->d_Ka = e->mem.alloc<double>(ne);
     e->d_Kb = e->mem.alloc<double>(ne);
@@ -1266,7 +1314,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, e->scan_tmp_bytes, e->d_counts, e->d_offs,
                                             (int)(e->n_slots + 1), e->stream));
     e->d_scan_tmp = e->mem.alloc<char>(e->scan_tmp_bytes);
-    e->d_stats = e->mem.alloc<unsigned long long>(4 * N_COMBOS);
+    e->d_stats = e->mem.alloc<unsigned long long>(6 * N_COMBOS);
     e->d_ranges = e->mem.alloc<long long>(4 * N_COMBOS);
     e->d_chunk_ctr = e->mem.alloc<unsigned int>(2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
     for (int m = 0; m < n_mol; ++m) e->max_N = std::max(e->max_N, e->mol_N[m]);
@@ -1317,10 +1365,11 @@ int enqueue_screen(hfx_engine* e, const BuildParams& P, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(std::max<long long>(1, std::min<long long>(1024, (e->n_slots + 256) / 256))),
                        dim3(256), 0, s, e->d_counts, (int)(e->n_slots + 1));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 4 * N_COMBOS);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, s, e->d_stats, 6 * N_COMBOS);
     ScreenArgs a{};
     a.cls = e->d_screen_cls;
     a.pair_info = e->d_pair_info;
+    a.pair_npx = e->d_pair_npx;
     a.pair_mol = e->d_pair_mol;
     a.pair_q = e->d_pair_q;
     a.sh_qbase = e->d_sh_qbase;
@@ -1400,7 +1449,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     const long long nt = e->n_telems;
     {
         const int g = e->n_cu * 4;
-        const long long nc = nt * hfx_engine::ACC_COPIES;
+        const long long nc = nt * hfx_engine::ACC_COPIES * (P.det ? 2 : 1);
         hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AJ, nc, 0.0);
         hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKa, nc, 0.0);
         if (uhf) hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, s, e->d_AKb, nc, 0.0);
@@ -1417,8 +1466,9 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     ea.AJ = e->d_AJ;
     ea.AKa = e->d_AKa;
     ea.AKb = e->d_AKb;
-    ea.acc_stride = nt;
+    ea.acc_stride = P.det ? 2 * nt : nt;
     ea.n_copies = hfx_engine::ACC_COPIES;
+    ea.fixed = P.det ? 1 : 0;
     ea.tauS = P.tauS;
     ea.out = nullptr;
     ea.out_off = nullptr;
@@ -1435,6 +1485,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     // LDS exchange rows of a class kernel: NA + NB rows of max_N doubles (twice for UHF), next
     // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without.
     auto tile_cols = [&](int k, int fp32) {
+        if (P.det) return 0;  // LDS atomics would add the exchange rows in a run-dependent order
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         const int la = pc_la(p1), lb = pc_lb(p1), L = la + lb + pc_la(p2) + pc_lb(p2);
         const size_t boys = P.stored ? 16 : (size_t)md::boys_points(!fp32) * md::boys_slice_nord(L, !fp32) * (fp32 ? 4 : 8);
@@ -1471,7 +1522,9 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
             for (int i = 1; i < n_str; ++i)
                 if (load[i] < load[best]) best = i;
             load[best] += it.cost;
-            for (int ch = it.b; ch < it.e; ++ch) cs[it.slot][ch] = e->side[best];
+            // slot 0 is the stream the build is enqueued on (the engine's own for hfx_jk, the
+            // caller's for hfx_jk_async); slots 1.. are the engine's side streams, forked from it
+            for (int ch = it.b; ch < it.e; ++ch) cs[it.slot][ch] = best == 0 ? s : e->side[best];
         }
     }
     if (n_str > 1) {
@@ -1513,10 +1566,12 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     {
         const dim3 g(std::max(1, std::min(e->n_cu * 4, (int)((ne + 255) / 256))), e->n_mol);
         const int cp = hfx_engine::ACC_COPIES;
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AJ, nt, cp, e->d_J, e->d_mbase, e->d_tbase, e->d_mn);
-        hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKa, nt, cp, e->d_Ka, e->d_mbase, e->d_tbase, e->d_mn);
+        auto symm = P.det ? k_symm_fixed : k_symm;
+        const long long st = ea.acc_stride;
+        hipLaunchKernelGGL(symm, g, dim3(256), 0, s, e->d_AJ, st, cp, e->d_J, e->d_mbase, e->d_tbase, e->d_mn);
+        hipLaunchKernelGGL(symm, g, dim3(256), 0, s, e->d_AKa, st, cp, e->d_Ka, e->d_mbase, e->d_tbase, e->d_mn);
         if (uhf)
-            hipLaunchKernelGGL(k_symm, g, dim3(256), 0, s, e->d_AKb, nt, cp, e->d_Kb, e->d_mbase, e->d_tbase, e->d_mn);
+            hipLaunchKernelGGL(symm, g, dim3(256), 0, s, e->d_AKb, st, cp, e->d_Kb, e->d_mbase, e->d_tbase, e->d_mn);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipGetLastError());
@@ -1538,18 +1593,70 @@ int use_q(hfx_engine* e, const double* Q, const char* who) {
     return HFX_OK;
 }
 
-int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
+// The engine's captured build for parameters P (captured again when they change).
+int ensure_graph(hfx_engine* e, const BuildParams& P) {
+    if (e->graph_exec && e->graph_key == P) return HFX_OK;
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    int ev_slot[2 * N_COMBOS];
+    int n_ev = 0;
+    hipStream_t s = e->stream;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_build(e, P, s, ev_slot, n_ev);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return fail(HFX_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+    e->graph = g;
+    HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+    e->graph_key = P;
+    return HFX_OK;
+}
+
+// Checks shared by hfx_jk and hfx_jk_async; fills the build parameters.
+int jk_check(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, bool want_hash,
+             const char* who, BuildParams& P) {
     if (!e || !args || !args->dD_tot || !args->dD_alpha || !J || !Ka)
-        return fail(HFX_ERR_INVALID, "hfx_jk: null argument");
+        return fail(HFX_ERR_INVALID, std::string(who) + ": null argument");
     const bool uhf = args->dD_beta != nullptr;
-    if (uhf && !Kb) return fail(HFX_ERR_INVALID, "hfx_jk: dD_beta given without Kb");
+    if (uhf && !Kb) return fail(HFX_ERR_INVALID, std::string(who) + ": dD_beta given without Kb");
     if (args->precision != HFX_FP64 && args->precision != HFX_MIXED)
-        return fail(HFX_ERR_INVALID, "hfx_jk: unknown precision");
+        return fail(HFX_ERR_INVALID, std::string(who) + ": unknown precision");
     const bool stored = (args->flags & HFX_JK_STORED) != 0;
     if (stored && !e->d_store)
-        return fail(HFX_ERR_INVALID, "hfx_jk: HFX_JK_STORED without stored integrals (call hfx_store_eris first)");
+        return fail(HFX_ERR_INVALID, std::string(who) + ": HFX_JK_STORED without stored integrals (call hfx_store_eris first)");
     if (stored && args->precision != HFX_FP64)
-        return fail(HFX_ERR_UNSUPPORTED, "hfx_jk: the stored integrals are FP64; HFX_JK_STORED needs HFX_FP64");
+        return fail(HFX_ERR_UNSUPPORTED, std::string(who) + ": the stored integrals are FP64; HFX_JK_STORED needs HFX_FP64");
+    if (!(args->schwarz_thresh >= 0.0) || !(args->density_thresh >= 0.0) || !(args->fp32_cut >= 0.0))
+        return fail(HFX_ERR_INVALID, std::string(who) + ": negative or NaN threshold");
+    P = BuildParams{args->schwarz_thresh, args->density_thresh, args->fp32_cut,
+                    args->precision == HFX_MIXED && args->fp32_cut > 0.0, uhf,
+                    (args->flags & HFX_JK_CLASS_TIMING) != 0};
+    P.hash = want_hash;
+    P.det = (args->flags & HFX_JK_DETERMINISTIC) != 0;
+    if (stored) {  // no density test, no FP32 split: the stored list as it is
+        P.tauD = 0.0;
+        P.cut = 0.0;
+        P.mixed = false;
+        P.hash = false;
+        P.stored = true;
+    }
+    return HFX_OK;
+}
+
+int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
+    BuildParams P;
+    {
+        const int rc = jk_check(e, args, J, Ka, Kb, st != nullptr && !(args && (args->flags & HFX_JK_NO_STATS)),
+                                "hfx_jk", P);
+        if (rc) return rc;
+    }
+    const bool uhf = P.uhf, stored = P.stored;
     const bool devp = (args->flags & HFX_JK_DEVICE_PTRS) != 0;
     const long long ne = e->n_elems;
     if (!devp) {
@@ -1568,39 +1675,12 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     HIPCHK(hipMemcpyAsync(e->d_Da, args->dD_alpha, ne * sizeof(double), h2d, s));
     if (uhf) HIPCHK(hipMemcpyAsync(e->d_Db, args->dD_beta, ne * sizeof(double), h2d, s));
     HIPCHK(hipEventRecord(e->ev0, s));
-    BuildParams P{args->schwarz_thresh, args->density_thresh, args->fp32_cut,
-                  args->precision == HFX_MIXED && args->fp32_cut > 0.0, uhf,
-                  (args->flags & HFX_JK_CLASS_TIMING) != 0};
-    P.hash = st != nullptr && !(args->flags & HFX_JK_NO_STATS);
-    if (stored) {  // no density test, no FP32 split: the stored list as it is
-        P.tauD = 0.0;
-        P.cut = 0.0;
-        P.mixed = false;
-        P.hash = false;
-        P.stored = true;
-    }
     int ev_slot[2 * N_COMBOS];
     int n_ev = 0;
     const bool timing = P.timing;
     if ((args->flags & HFX_JK_USE_GRAPH) && !timing) {
-        if (!e->graph_exec || !(e->graph_key == P)) {
-            if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-            if (e->graph) (void)hipGraphDestroy(e->graph);
-            e->graph_exec = nullptr;
-            e->graph = nullptr;
-            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_build(e, P, s, ev_slot, n_ev);
-            hipGraph_t g = nullptr;
-            const hipError_t ce = hipStreamEndCapture(s, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
-            if (ce != hipSuccess) return fail(HFX_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
-            e->graph = g;
-            HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
-            e->graph_key = P;
-        }
+        const int rc = ensure_graph(e, P);
+        if (rc) return rc;
         HIPCHK(hipGraphLaunch(e->graph_exec, s));
     } else {
         const int rc = enqueue_build(e, P, s, ev_slot, n_ev);
@@ -1611,7 +1691,7 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
     HIPCHK(hipMemcpyAsync(Ka, e->d_Ka, ne * sizeof(double), d2h, s));
     if (uhf) HIPCHK(hipMemcpyAsync(Kb, e->d_Kb, ne * sizeof(double), d2h, s));
     long long rng[4 * N_COMBOS];
-    unsigned long long cnt[2 * N_COMBOS], stv[4 * N_COMBOS];
+    unsigned long long cnt[2 * N_COMBOS], stv[6 * N_COMBOS];
     const bool want_stats = st && !(args->flags & HFX_JK_NO_STATS) && !stored;
     if (want_stats) {
         HIPCHK(hipMemcpyAsync(rng, e->d_ranges, sizeof(rng), hipMemcpyDeviceToHost, s));
@@ -1628,23 +1708,27 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
                 cnt[2 * k] = (unsigned long long)(rng[2 * k + 1] - rng[2 * k]);
                 cnt[2 * k + 1] = (unsigned long long)(rng[2 * N_COMBOS + 2 * k + 1] - rng[2 * N_COMBOS + 2 * k]);
             }
-            double flop = 0.0;
+            double flop = 0.0, xflop = 0.0;
             for (int k = 0; k < N_COMBOS; ++k) {
                 const long long c = (long long)(cnt[2 * k] + cnt[2 * k + 1]);
                 st->class_count[k] = c;
                 st->n_quartets += c;
                 st->n_fp32 += (long long)cnt[2 * k + 1];
-                st->hash_sum += stv[4 * k];
-                st->hash_xor ^= stv[4 * k + 1];
+                st->hash_sum += stv[6 * k];
+                st->hash_xor ^= stv[6 * k + 1];
                 const int p1 = combo_pc1(k), p2 = combo_pc2(k);
                 const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
                 const double f = model_flop_h(la, lb, lc, ld);
                 const double dig = 12.0 * ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
-                st->class_gflop64[k] = ((double)stv[4 * k + 2] * f + dig * (double)cnt[2 * k]) * 1e-9;
-                st->class_gflop32[k] = ((double)stv[4 * k + 3] * f + dig * (double)cnt[2 * k + 1]) * 1e-9;
+                st->class_gflop64[k] = ((double)stv[6 * k + 2] * f + dig * (double)cnt[2 * k]) * 1e-9;
+                st->class_gflop32[k] = ((double)stv[6 * k + 3] * f + dig * (double)cnt[2 * k + 1]) * 1e-9;
+                st->class_exec_gflop64[k] = ((double)stv[6 * k + 4] * f + dig * (double)cnt[2 * k]) * 1e-9;
+                st->class_exec_gflop32[k] = ((double)stv[6 * k + 5] * f + dig * (double)cnt[2 * k + 1]) * 1e-9;
                 flop += st->class_gflop64[k] + st->class_gflop32[k];
+                xflop += st->class_exec_gflop64[k] + st->class_exec_gflop32[k];
             }
             st->model_gflop = flop;
+            st->exec_gflop = xflop;
         }
         if (stored && !(args->flags & HFX_JK_NO_STATS)) {  // the stored list; digest FLOPs only
             double flop = 0.0;
@@ -1653,10 +1737,10 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
                 const double dig = 12.0 * ncart_h(pc_la(p1)) * ncart_h(pc_lb(p1)) * ncart_h(pc_la(p2)) * ncart_h(pc_lb(p2));
                 st->class_count[k] = e->store_n[k];
                 st->n_quartets += e->store_n[k];
-                st->class_gflop64[k] = dig * (double)e->store_n[k] * 1e-9;
+                st->class_gflop64[k] = st->class_exec_gflop64[k] = dig * (double)e->store_n[k] * 1e-9;
                 flop += st->class_gflop64[k];
             }
-            st->model_gflop = flop;
+            st->model_gflop = st->exec_gflop = flop;
         }
         if (timing) {
             int prev = 0;
@@ -1668,6 +1752,64 @@ int jk_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, doubl
                 prev = ev_slot[x];
             }
         }
+    }
+    return HFX_OK;
+}
+
+// hfx_jk_async: the build of jk_impl enqueued on the caller's stream, nothing waited for.
+int jk_async_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hipStream_t s) {
+    BuildParams P;
+    {
+        const int rc = jk_check(e, args, J, Ka, Kb, false, "hfx_jk_async", P);
+        if (rc) return rc;
+    }
+    if (!(args->flags & HFX_JK_DEVICE_PTRS))
+        return fail(HFX_ERR_INVALID, "hfx_jk_async: needs HFX_JK_DEVICE_PTRS (device-resident densities and outputs)");
+    if (P.timing) return fail(HFX_ERR_INVALID, "hfx_jk_async: HFX_JK_CLASS_TIMING needs the blocking hfx_jk");
+    HIPCHK(hipSetDevice(e->device));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(s, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    if (!P.stored) {
+        const bool resort = args->Q ? (e->sort_q.size() != (size_t)e->n_qelems ||
+                                       std::memcmp(e->sort_q.data(), args->Q, e->n_qelems * sizeof(double)) != 0)
+                                    : (e->sort_q != e->own_q);
+        if (resort && capturing)
+            return fail(HFX_ERR_INVALID, "hfx_jk_async: a new Schwarz table re-sorts the pair lists synchronously; "
+                                         "make one call with it before capturing");
+        const int rc = use_q(e, args->Q, "hfx_jk_async");
+        if (rc) return rc;
+    }
+    const bool graph = (args->flags & HFX_JK_USE_GRAPH) != 0;
+    if (graph && capturing)
+        return fail(HFX_ERR_INVALID, "hfx_jk_async: HFX_JK_USE_GRAPH on a capturing stream (capture the call without it)");
+    if (graph) {
+        const int rc = ensure_graph(e, P);
+        if (rc) return rc;
+    }
+    if (!capturing) {  // after the engine's earlier work
+        HIPCHK(hipEventRecord(e->ev_async, e->stream));
+        HIPCHK(hipStreamWaitEvent(s, e->ev_async, 0));
+    }
+    const long long ne = e->n_elems;
+    const hipMemcpyKind d2d = hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(e->d_Dt, args->dD_tot, ne * sizeof(double), d2d, s));
+    HIPCHK(hipMemcpyAsync(e->d_Da, args->dD_alpha, ne * sizeof(double), d2d, s));
+    if (P.uhf) HIPCHK(hipMemcpyAsync(e->d_Db, args->dD_beta, ne * sizeof(double), d2d, s));
+    if (graph) {
+        HIPCHK(hipGraphLaunch(e->graph_exec, s));
+    } else {
+        int ev_slot[2 * N_COMBOS];
+        int n_ev = 0;
+        const int rc = enqueue_build(e, P, s, ev_slot, n_ev);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(J, e->d_J, ne * sizeof(double), d2d, s));
+    HIPCHK(hipMemcpyAsync(Ka, e->d_Ka, ne * sizeof(double), d2d, s));
+    if (P.uhf) HIPCHK(hipMemcpyAsync(Kb, e->d_Kb, ne * sizeof(double), d2d, s));
+    if (!capturing) {  // the engine's later calls wait for this build
+        HIPCHK(hipEventRecord(e->ev_async, s));
+        HIPCHK(hipStreamWaitEvent(e->stream, e->ev_async, 0));
     }
     return HFX_OK;
 }
@@ -2099,6 +2241,10 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st);
+}
+
+int hfx_jk_async(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, void* stream) {
+    return jk_async_impl(e, args, J, Ka, Kb, static_cast<hipStream_t>(stream));
 }
 
 int hfx_store_eris(hfx_engine* e, const double* Q, double schwarz_thresh, int64_t* n_quartets, int64_t* n_bytes) {

@@ -77,6 +77,7 @@ struct EriArgs {
     double* AKb;   // register); element (i,j) holds the sum of both orientations
     long long acc_stride;
     int n_copies;
+    int fixed;     // 1: accumulators hold fixed-point pairs (element t at [2t, 2t+1], int64 each)
     double tauS;
     double* out;               // store mode (non-NULL): blocks [ab][cd], canonical class order
     const long long* out_off;  // store mode: offset of each entry's block (NULL: the stored-integral

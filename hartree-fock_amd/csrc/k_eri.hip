@@ -24,7 +24,8 @@
 // dropped as the reference's AO test (src/SCF.cpp:670-671).  Contributions go to
 //   AJ[i,j] += 2 s I D[k,l],  AJ[k,l] += 2 s I D[i,j],
 //   AK[i,k] += s I D[j,l], AK[i,l] += s I D[j,k], AK[j,k] += s I D[i,l], AK[j,l] += s I D[i,k]
-// accumulated per thread in registers and flushed with FP64 atomics into packed lower
+// accumulated per thread in registers and flushed with FP64 atomics (or, in the deterministic
+// mode, two-word fixed-point integer atomics) into packed lower
 // triangles (both orientations of (i,j) land in one element, i.e. J = AJ + AJ^T is formed
 // on the fly; the diagonal is doubled when the triangles are expanded).  This equals the reference's canonical-AO digestion
 // (src/SCF.cpp:633-711) summed over the 8 permutations.
@@ -83,9 +84,27 @@ __device__ __forceinline__ const double* boys_of<double>(const EriArgs& a) { ret
 template <>
 __device__ __forceinline__ const float* boys_of<float>(const EriArgs& a) { return a.boys32; }
 
-// Accumulation into the copy private to this XCD: no other XCD touches these lines.
-__device__ __forceinline__ void atomic_add(double* p, double v) {
-    if (v != 0.0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Accumulation into the copy private to this XCD (no other XCD touches these lines): FP64
+// atomics, or in the deterministic mode the fixed-point pairs below.
+// Deterministic accumulation: v as a two-word fixed-point number, hi = round(v 2^32) and
+// lo = round((v 2^32 - hi) 2^40), each added with an integer atomic.  Integer sums do not
+// depend on the order of the additions, so the result is bitwise reproducible; the rounding
+// error of one term is at most 2^-73 and the range is |sum| < 2^31.
+__device__ __forceinline__ void fixed_add(unsigned long long* q, double v) {
+    const double x = ldexp(v, 32);
+    const double r = rint(x);
+    const long long hi = (long long)r;
+    const long long lo = (long long)rint(ldexp(x - r, 40));
+    if (hi) __hip_atomic_fetch_add(q, (unsigned long long)hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lo) __hip_atomic_fetch_add(q + 1, (unsigned long long)lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Element t of an accumulator block: a double (fx = 0) or a fixed-point pair at [2t, 2t+1] (fx = 1).
+__device__ __forceinline__ void acc_add(double* base, long long t, double v, int fx) {
+    if (v == 0.0) return;
+    if (!fx)
+        __hip_atomic_fetch_add(base + t, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        fixed_add(reinterpret_cast<unsigned long long*>(base) + 2 * t, v);
 }
 // Accumulation into the block's LDS exchange rows.
 __device__ __forceinline__ void lds_add(double* p, double v) {
@@ -334,7 +353,8 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             return (fabs(v) < tauS) ? 0.0 : v * s;
         };
         const long long copy = (long long)(xcd_id() % a.n_copies) * a.acc_stride;
-        double* AJ = a.AJ + copy + base;
+        const int fx = a.fixed;
+        double* AJ = a.AJ + copy + (base << fx);
         {
             double Jab[CB], Jcd[NCD];
             sfor<CB>([&](auto x) { Jab[x] = 0.0; });
@@ -379,12 +399,12 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                         const double o = __shfl_down(x, off);
                         if (same[decltype(k_)::value]) x += o;
                     });
-                    if (head && valid) atomic_add(&AJ[tri_index(oA + ab / NB, oB + ab % NB)], 2.0 * x);
+                    if (head && valid) acc_add(AJ, tri_index(oA + ab / NB, oB + ab % NB), 2.0 * x, fx);
                 }
             });
             sfor<NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
-                atomic_add(&AJ[tri_index(oC + cd / ND, oD + cd % ND)], 2.0 * Jcd[cd]);
+                acc_add(AJ, tri_index(oC + cd / ND, oD + cd % ND), 2.0 * Jcd[cd], fx);
             });
         }
         // Exchange with one spin density Dx into the accumulator AKx (rows of a and b into the
@@ -439,11 +459,11 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                     } else {
                         sfor<NC>([&](auto jc_) {
                             constexpr int jc = decltype(jc_)::value;
-                            atomic_add(&AKx[tri_index(orow + r, oC + jc)], Kc[r][jc]);
+                            acc_add(AKx, tri_index(orow + r, oC + jc), Kc[r][jc], fx);
                         });
                         sfor<ND>([&](auto jd_) {
                             constexpr int jd = decltype(jd_)::value;
-                            atomic_add(&AKx[tri_index(orow + r, oD + jd)], Kd[r][jd]);
+                            acc_add(AKx, tri_index(orow + r, oD + jd), Kd[r][jd], fx);
                         });
                     }
                 }
@@ -456,8 +476,8 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             exchange_side(md::ic<1>{}, Dx, AKx, T);
             __builtin_amdgcn_sched_barrier(0);
         };
-        exchange(Da, a.AKa + copy + base, tile);
-        if (Db) exchange(Db, a.AKb + copy + base, tile ? tile + (NA + NB) * tn : nullptr);
+        exchange(Da, a.AKa + copy + (base << fx), tile);
+        if (Db) exchange(Db, a.AKb + copy + (base << fx), tile ? tile + (NA + NB) * tn : nullptr);
         if (in_tile && valid) {
             unsigned int* mask = (unsigned int*)(tile + (Db ? 2 : 1) * (NA + NB) * tn);
             const int lc = a.sh_local[sc], ld = a.sh_local[sd];
@@ -477,6 +497,10 @@ __device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles)
     const int tn = a.tile_n;
     const int oA = a.sh_ao[sa], oB = a.sh_ao[sb];
     const int g0 = sa - a.sh_local[sa], ns = a.sh_ns[sa];
+    // Never reached in the fixed-point mode (the engine keeps no LDS rows then), so the rows go
+    // out as plain FP64 atomics.  (A two-branch acc_add here was miscompiled on gfx950: builds
+    // with it gave wrong, run-dependent exchange rows; tests/test_gpu_async_det.py compares the
+    // two accumulation modes on every class.)
     const long long copy = (long long)(xcd_id() % a.n_copies) * a.acc_stride + a.sh_tbase[sa];
     unsigned int* mask = (unsigned int*)(tile + n_tiles * (NA + NB) * tn);
     for (int sh = threadIdx.x; sh < ns; sh += blockDim.x) {
@@ -488,8 +512,10 @@ __device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles)
             for (int r = 0; r < NA + NB; ++r) {
                 const int row = r < NA ? oA + r : oB + (r - NA);
                 for (int c = 0; c < nc; ++c) {
-                    atomic_add(&AK[tri_index(row, o + c)], T[r * tn + o + c]);
+                    const double v = T[r * tn + o + c];
                     T[r * tn + o + c] = 0.0;
+                    if (v != 0.0)
+                        __hip_atomic_fetch_add(&AK[tri_index(row, o + c)], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
         }

@@ -23,6 +23,15 @@ def _lib():
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipSetDevice.argtypes = [C.c_int]
         _hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        _hip.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        _hip.hipStreamDestroy.argtypes = [C.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+        _hip.hipStreamBeginCapture.argtypes = [C.c_void_p, C.c_int]
+        _hip.hipStreamEndCapture.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        _hip.hipGraphInstantiate.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+        _hip.hipGraphLaunch.argtypes = [C.c_void_p, C.c_void_p]
+        _hip.hipGraphExecDestroy.argtypes = [C.c_void_p]
+        _hip.hipGraphDestroy.argtypes = [C.c_void_p]
     return _hip
 
 
@@ -91,3 +100,51 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed: %d" % (what, rc))
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority (for hfx_jk_async callers)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        _ok(_lib().hipStreamCreateWithFlags(C.byref(self.h), 1), "hipStreamCreateWithFlags")
+
+    def synchronize(self):
+        _ok(_lib().hipStreamSynchronize(self.h), "hipStreamSynchronize")
+
+    def begin_capture(self):
+        _ok(_lib().hipStreamBeginCapture(self.h, 1), "hipStreamBeginCapture")  # thread-local mode
+
+    def end_capture(self):
+        """Ends the capture and returns the instantiated graph."""
+        g = C.c_void_p()
+        _ok(_lib().hipStreamEndCapture(self.h, C.byref(g)), "hipStreamEndCapture")
+        return Graph(g)
+
+    def destroy(self):
+        if self.h:
+            _lib().hipStreamDestroy(self.h)
+            self.h = None
+
+
+class Graph:
+    """An instantiated graph captured from a Stream."""
+
+    def __init__(self, g):
+        self.g = g
+        self.x = C.c_void_p()
+        _ok(_lib().hipGraphInstantiate(C.byref(self.x), g, None, None, 0), "hipGraphInstantiate")
+
+    def launch(self, stream):
+        _ok(_lib().hipGraphLaunch(self.x, stream.h), "hipGraphLaunch")
+
+    def destroy(self):
+        if self.x:
+            _lib().hipGraphExecDestroy(self.x)
+            _lib().hipGraphDestroy(self.g)
+            self.x = None

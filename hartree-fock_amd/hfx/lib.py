@@ -26,6 +26,7 @@ HFX_JK_USE_GRAPH = 2
 HFX_JK_NO_STATS = 4
 HFX_JK_CLASS_TIMING = 8
 HFX_JK_STORED = 16
+HFX_JK_DETERMINISTIC = 32
 N_CLASSES = 21
 
 ERRORS = {1: "invalid argument", 2: "unsupported", 3: "out of memory", 4: "device error",
@@ -53,14 +54,17 @@ class HfxJkStats(C.Structure):
                 ("hash_sum", C.c_uint64), ("hash_xor", C.c_uint64), ("model_gflop", C.c_double),
                 ("device_ms", C.c_double), ("class_gflop64", C.c_double * N_CLASSES),
                 ("class_gflop32", C.c_double * N_CLASSES), ("class_ms64", C.c_double * N_CLASSES),
-                ("class_ms32", C.c_double * N_CLASSES)]
+                ("class_ms32", C.c_double * N_CLASSES), ("exec_gflop", C.c_double),
+                ("class_exec_gflop64", C.c_double * N_CLASSES), ("class_exec_gflop32", C.c_double * N_CLASSES)]
 
     def as_dict(self):
         return dict(n_quartets=self.n_quartets, n_fp32=self.n_fp32, class_count=list(self.class_count),
                     hash_sum=self.hash_sum, hash_xor=self.hash_xor, model_gflop=self.model_gflop,
                     device_ms=self.device_ms, class_gflop64=list(self.class_gflop64),
                     class_gflop32=list(self.class_gflop32), class_ms64=list(self.class_ms64),
-                    class_ms32=list(self.class_ms32))
+                    class_ms32=list(self.class_ms32), exec_gflop=self.exec_gflop,
+                    class_exec_gflop64=list(self.class_exec_gflop64),
+                    class_exec_gflop32=list(self.class_exec_gflop32))
 
 
 class HfxGradArgs(C.Structure):
@@ -98,7 +102,7 @@ class HfxScfResult(C.Structure):
 
 
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
-               "hfx_nuclear", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk", "hfx_last_error",
+               "hfx_nuclear", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk", "hfx_jk_async", "hfx_last_error",
                "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine", "hfx_scf_run_batched",
                "hfx_scf_run_batched_engine", "hfx_rhf_gradient", "hfx_store_eris", "hfx_release_eris"]
 
@@ -130,6 +134,8 @@ class Library:
         d.hfx_nuclear.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _i32p, _f64p]
         d.hfx_eri_quartets.argtypes = [C.c_void_p, C.c_int64, _i32p, C.POINTER(C.c_int64), _f64p]
         d.hfx_jk.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.POINTER(HfxJkStats)]
+        if hasattr(d, "hfx_jk_async"):  # (a library built before this entry point lacks it)
+            d.hfx_jk_async.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.c_void_p]
         d.hfx_store_eris.argtypes = [C.c_void_p, _f64p, C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         d.hfx_release_eris.argtypes = [C.c_void_p]
         d.hfx_rhf_gradient.argtypes = [C.c_void_p, C.POINTER(HfxGradArgs), _f64p, _f64p, C.POINTER(HfxGradStats)]
@@ -345,6 +351,20 @@ def jk_device(engine, dD_tot, dD_alpha, J, Ka, dD_beta=None, Kb=None, Q=None, sc
     st = HfxJkStats()
     engine.lib.check(engine.lib.dll.hfx_jk(engine.h, C.byref(a), vp(J), vp(Ka), vp(Kb), C.byref(st)), "hfx_jk")
     return st.as_dict()
+
+
+def jk_async(engine, dD_tot, dD_alpha, J, Ka, stream, dD_beta=None, Kb=None, Q=None, schwarz_thresh=1e-10,
+             density_thresh=1e-10, precision=HFX_FP64, fp32_cut=0.0, flags=0):
+    """hfx_jk_async on device buffers (hfx.device.DeviceArray) and a HIP stream (hfx.device.Stream,
+    a c_void_p handle, or None for the null stream): enqueued, not waited for."""
+    a = HfxJkArgs()
+    vp = lambda d: None if d is None else C.cast(d.ptr, _f64p)
+    a.dD_tot, a.dD_alpha, a.dD_beta = vp(dD_tot), vp(dD_alpha), vp(dD_beta)
+    a.Q = None if Q is None else _p(Q)
+    a.schwarz_thresh, a.density_thresh = schwarz_thresh, density_thresh
+    a.precision, a.fp32_cut, a.flags = precision, fp32_cut, flags | HFX_JK_DEVICE_PTRS
+    h = getattr(stream, "h", stream)  # hfx.device.Stream, a c_void_p or None
+    engine.lib.check(engine.lib.dll.hfx_jk_async(engine.h, C.byref(a), vp(J), vp(Ka), vp(Kb), h), "hfx_jk_async")
 
 
 def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, outputs=True, trace_cap=200):

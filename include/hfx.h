@@ -40,9 +40,15 @@ enum {
     HFX_JK_USE_GRAPH = 2,   /* replay the engine's captured hipGraph of the build pipeline   */
     HFX_JK_NO_STATS = 4,    /* skip the survivor statistics read-back                         */
     HFX_JK_CLASS_TIMING = 8, /* time every class kernel group with HIP events (stats.class_ms*) */
-    HFX_JK_STORED = 16       /* digest the integrals kept by hfx_store_eris: no density test, FP64 only,
+    HFX_JK_STORED = 16,      /* digest the integrals kept by hfx_store_eris: no density test, FP64 only,
                               * args->Q and density_thresh unused (the stored list is fixed);
                               * the conventional build SCF::buildFockMatrix (src/SCF.cpp:233-521) */
+    HFX_JK_DETERMINISTIC = 32 /* bitwise-reproducible J/K: every contribution is added as a 2 x 64-bit
+                              * fixed-point integer (resolution 2^-72, range +-2^31), so the sum does not
+                              * depend on the order in which the device performs the additions; slower
+                              * (no LDS exchange rows, two integer atomics per term).  The reference's
+                              * critical-section reduction (src/SCF.cpp:717-724) is order-dependent too;
+                              * this mode has no counterpart there. */
 };
 
 #define HFX_MAX_L 2
@@ -102,6 +108,12 @@ typedef struct hfx_jk_stats {
     double class_gflop32[HFX_N_CLASSES];
     double class_ms64[HFX_N_CLASSES];     /* HFX_JK_CLASS_TIMING: device ms of each part  */
     double class_ms32[HFX_N_CLASSES];
+    /* The same model FLOP count over the primitive quartets the kernels actually execute: the
+     * engine leaves out primitive pairs whose |K c_a c_b N_a N_b| is below 1e-20 (FP64 work) or
+     * 1e-11 (FP32 work), which model_gflop above still counts. */
+    double exec_gflop;
+    double class_exec_gflop64[HFX_N_CLASSES];
+    double class_exec_gflop32[HFX_N_CLASSES];
 } hfx_jk_stats;
 
 /* IntegralEngine::IntegralEngine (src/IntegralEngine.cpp:13-75).  Builds the device basis,
@@ -148,6 +160,25 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
  * F = F_prev + J - K (src/SCF.cpp:731).  Kb may be NULL for RHF.  stats may be NULL. */
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb,
            hfx_jk_stats* stats);
+
+/* hfx_jk enqueued on the caller's HIP stream without waiting for it (the graph-capturable
+ * variant of the build): `stream` is a hipStream_t passed as void* (NULL: the null stream).
+ * For a caller that overlaps the build with its own work, or captures a whole SCF iteration
+ * (SCF::run calls the build at src/SCF.cpp:57-58) into its own hipGraph.
+ *   - args->flags must include HFX_JK_DEVICE_PTRS: the densities and J/K are device buffers that
+ *     stay valid until the stream has passed the end of the build; HFX_JK_CLASS_TIMING is refused
+ *     and no statistics are returned (they would need a host read-back).
+ *   - The call records no host synchronization and no memset; the build runs on `stream` and on
+ *     the engine's side streams, forked from and joined back into `stream`.
+ *   - HFX_JK_USE_GRAPH launches the engine's own captured build on `stream`; refused while
+ *     `stream` is capturing (capture the call without it instead).
+ *   - A Schwarz table that differs from the previous call's re-sorts the engine's pair lists
+ *     synchronously, which a capturing stream refuses (HFX_ERR_INVALID): make one call with the
+ *     table before capturing.
+ *   - Calls on one engine must not overlap: outside capture the engine orders this build after
+ *     its earlier work and its later calls after this build; a captured build must not be
+ *     replayed while another call on the same engine runs. */
+int hfx_jk_async(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, void* stream);
 
 /* Conventional (stored-integral) mode.  Computes once, and keeps in HBM, the normalized
  * integral blocks of every canonical shell quartet with Q(a,b)Q(c,d) >= schwarz_thresh (Q as

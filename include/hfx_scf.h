@@ -84,7 +84,11 @@ int hfx_scf_run(const hfx_basis* basis, int32_t n_atoms, const double* Z, const 
                 int32_t charge, int32_t multiplicity, const hfx_scf_options* opt, int device,
                 hfx_scf_result* res);
 
-/* The same loop over an engine the caller created (the engine is not destroyed). */
+/* The same loop over an engine the caller created (the engine is not destroyed).  With
+ * opt->direct = 0 the run stores its own integrals on that engine (hfx_store_eris with the
+ * engine's Schwarz table and opt->schwarz_thresh), replacing any the caller stored before,
+ * and releases them when it returns: a caller that keeps stored integrals on an engine should
+ * not pass it to a conventional run. */
 int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz,
                        int32_t charge, int32_t multiplicity, const hfx_scf_options* opt,
                        hfx_scf_result* res);
@@ -101,7 +105,8 @@ int hfx_scf_run_batched(int32_t n_mol, const hfx_basis* bases, int32_t n_atoms, 
                         const int32_t* multiplicity, const hfx_scf_options* opt, int device,
                         hfx_scf_result* res);
 
-/* The same over a batched engine the caller created; n_aos_mol[m] = AOs of molecule m. */
+/* The same over a batched engine the caller created; n_aos_mol[m] = AOs of molecule m.
+ * opt->direct = 0 replaces and then releases the engine's stored integrals, as above. */
 int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_aos_mol, int32_t n_atoms,
                                const double* Z, const double* xyz, const int32_t* atom_mol,
                                const int32_t* charge, const int32_t* multiplicity,
