@@ -659,6 +659,39 @@ struct DevArena {
 
 }  // namespace
 
+// Staging memory of hfx_one_electron_noalloc: the output blocks (SCRATCH_OUT doubles), the atoms'
+// charges and positions (SCRATCH_ATOMS of each) and the per-molecule atom ranges.  Every GPU holds
+// its own copy for the life of the process; host code uses the address hipGetSymbolAddress gives
+// for the current device (scratch_address), never the array's name.
+namespace {
+constexpr long long SCRATCH_OUT = 1LL << 21;  // 16 MiB of output staging
+constexpr int SCRATCH_ATOMS = 4096;
+constexpr int SCRATCH_MOLS = 4096;
+constexpr long long SCRATCH_DOUBLES = SCRATCH_OUT + 4LL * SCRATCH_ATOMS + (SCRATCH_MOLS + 2) / 2 + 1;
+}  // namespace
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (which must be current), looked up once per device.
+int scratch_address(int device, double** out) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return fail(HFX_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess || !p)
+            return fail(HFX_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return HFX_OK;
+}
+}  // namespace
+
 struct hfx_engine {
     int device = 0;
     int n_cu = 256;
@@ -671,6 +704,7 @@ struct hfx_engine {
     hipEvent_t ev_fork = nullptr, ev_join[N_SIDE] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_async = nullptr;  // orders hfx_jk_async builds against the engine's stream
+    double* scratch = nullptr;      // this device's address of the allocation-free staging array
     DevArena mem;
 
     // molecules
@@ -1062,6 +1096,10 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     HIPCHK(hipSetDevice(device));
     std::unique_ptr<hfx_engine> e(new hfx_engine());
     e->device = device;
+    {
+        const int rc = scratch_address(device, &e->scratch);
+        if (rc) return rc;
+    }
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     e->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -2085,6 +2123,40 @@ int hfx_sizes(const hfx_engine* e, int64_t* n_aos, int64_t* n_shells, int64_t* n
     return HFX_OK;
 }
 
+// The one-electron kernels of every pair class, writing the packed N_m x N_m blocks to dout; the
+// atoms of molecule m are dz/dxyz[da0[m] .. da0[m+1]) (device pointers).
+static int launch_one_electron(hfx_engine* e, int kind, double* dout, const double* dz, const double* dxyz,
+                               const int* da0) {
+    OneArgs o{};
+    o.pair_a = e->d_pair_a;
+    o.pair_b = e->d_pair_b;
+    o.sh_po = e->d_sh_po;
+    o.sh_np = e->d_sh_np;
+    o.sh_no = e->d_sh_no;
+    o.sh_ao = e->d_sh_ao;
+    o.sh_mbase = e->d_sh_mbase;
+    o.sh_mn = e->d_sh_mn;
+    o.sh_mol = e->d_sh_mol;
+    o.ctr = e->d_ctr;
+    o.alpha = e->d_alpha;
+    o.coeff = e->d_coeff;
+    o.norm = e->d_norm;
+    o.atom_z = dz;
+    o.atom_xyz = dxyz;
+    o.mol_atom0 = da0;
+    o.boys = e->d_boys64;
+    o.out = dout;
+    for (int pc = 0; pc < N_PAIR_CLASSES; ++pc) {
+        o.ids = e->d_cls_ids[pc];
+        o.n = (int)e->cls_ids[pc].size();
+        if (kind == 0) launch_one<0>(pc, o, e->stream);
+        else if (kind == 1) launch_one<1>(pc, o, e->stream);
+        else launch_one<2>(pc, o, e->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return HFX_OK;
+}
+
 static int one_electron(hfx_engine* e, int kind, int32_t nat, const double* Z, const double* xyz,
                         const int32_t* atom_mol, double* M) {
     if (!e || !M) return fail(HFX_ERR_INVALID, "null argument");
@@ -2118,35 +2190,68 @@ static int one_electron(hfx_engine* e, int kind, int32_t nat, const double* Z, c
         HIPCHK(hipMemcpy(dxyz, axyz.data(), axyz.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(da0, atom0.data(), atom0.size() * sizeof(int), hipMemcpyHostToDevice));
-    OneArgs o{};
-    o.pair_a = e->d_pair_a;
-    o.pair_b = e->d_pair_b;
-    o.sh_po = e->d_sh_po;
-    o.sh_np = e->d_sh_np;
-    o.sh_no = e->d_sh_no;
-    o.sh_ao = e->d_sh_ao;
-    o.sh_mbase = e->d_sh_mbase;
-    o.sh_mn = e->d_sh_mn;
-    o.sh_mol = e->d_sh_mol;
-    o.ctr = e->d_ctr;
-    o.alpha = e->d_alpha;
-    o.coeff = e->d_coeff;
-    o.norm = e->d_norm;
-    o.atom_z = dz;
-    o.atom_xyz = dxyz;
-    o.mol_atom0 = da0;
-    o.boys = e->d_boys64;
-    o.out = dout;
-    for (int pc = 0; pc < N_PAIR_CLASSES; ++pc) {
-        o.ids = e->d_cls_ids[pc];
-        o.n = (int)e->cls_ids[pc].size();
-        if (kind == 0) launch_one<0>(pc, o, e->stream);
-        else if (kind == 1) launch_one<1>(pc, o, e->stream);
-        else launch_one<2>(pc, o, e->stream);
+    {
+        const int rc = launch_one_electron(e, kind, dout, dz, dxyz, da0);
+        if (rc) return rc;
     }
-    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(M, dout, e->n_elems * sizeof(double), hipMemcpyDeviceToHost));
+    return HFX_OK;
+}
+
+// hfx_one_electron_noalloc: the same kernels with the staging of the atoms and of the output in
+// the device's `scratch` array (hfx_create looked its address up once per device).
+static int one_electron_noalloc(hfx_engine* e, int kind, int32_t nat, const double* Z, const double* xyz,
+                                const int32_t* atom_mol, double* M) {
+    if (!e || !M) return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: null argument");
+    if (kind < 0 || kind > 2) return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: kind must be 0, 1 or 2");
+    if (!e->scratch) return fail(HFX_ERR_DEVICE, "hfx_one_electron_noalloc: no scratch address for this device");
+    if (e->n_elems > SCRATCH_OUT)
+        return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: " + std::to_string(e->n_elems) +
+                                         " matrix elements exceed the allocation-free limit of " +
+                                         std::to_string(SCRATCH_OUT) + " (use hfx_overlap/hfx_kinetic/hfx_nuclear)");
+    if (e->n_mol > SCRATCH_MOLS)
+        return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: more than " + std::to_string(SCRATCH_MOLS) + " molecules");
+    const int na = kind == 2 ? nat : 0;
+    if (kind == 2) {
+        if (nat < 0 || (nat > 0 && (!Z || !xyz))) return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: bad atoms");
+        if (nat > SCRATCH_ATOMS)
+            return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: " + std::to_string(nat) +
+                                             " atoms exceed the allocation-free limit of " + std::to_string(SCRATCH_ATOMS));
+        if (e->n_mol > 1 && !atom_mol)
+            return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: atom_mol required for batched engines");
+        if (!finite_all(Z, nat) || !finite_all(xyz, 3LL * nat))
+            return fail(HFX_ERR_NONFINITE, "hfx_one_electron_noalloc: non-finite atoms");
+    }
+    // atom ranges per molecule; the atoms must come grouped by molecule, in molecule order
+    static thread_local int atom0[SCRATCH_MOLS + 1];
+    {
+        int a = 0;
+        for (int m = 0; m < e->n_mol; ++m) {
+            atom0[m] = a;
+            while (a < na && (!atom_mol || atom_mol[a] == m)) ++a;
+        }
+        atom0[e->n_mol] = a;
+        if (a != na)
+            return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: atoms must be grouped by molecule in molecule order");
+    }
+    double* dout = e->scratch;
+    double* dz = e->scratch + SCRATCH_OUT;
+    double* dxyz = dz + SCRATCH_ATOMS;
+    int* da0 = reinterpret_cast<int*>(dxyz + 3 * SCRATCH_ATOMS);
+    hipStream_t s = e->stream;
+    HIPCHK(hipSetDevice(e->device));
+    if (na > 0) {
+        HIPCHK(hipMemcpyAsync(dz, Z, na * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dxyz, xyz, 3 * (size_t)na * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipMemcpyAsync(da0, atom0, (e->n_mol + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    {
+        const int rc = launch_one_electron(e, kind, dout, dz, dxyz, da0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(M, dout, e->n_elems * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return HFX_OK;
 }
 
@@ -2154,6 +2259,11 @@ int hfx_overlap(hfx_engine* e, double* S) { return one_electron(e, 0, 0, nullptr
 int hfx_kinetic(hfx_engine* e, double* T) { return one_electron(e, 1, 0, nullptr, nullptr, nullptr, T); }
 int hfx_nuclear(hfx_engine* e, int32_t nat, const double* Z, const double* xyz, const int32_t* atom_mol, double* V) {
     return one_electron(e, 2, nat, Z, xyz, atom_mol, V);
+}
+
+int hfx_one_electron_noalloc(hfx_engine* e, int32_t kind, int32_t n_atoms, const double* Z, const double* xyz,
+                             const int32_t* atom_mol, double* M) {
+    return one_electron_noalloc(e, kind, n_atoms, Z, xyz, atom_mol, M);
 }
 
 int hfx_schwarz(hfx_engine* e, double* Q) {

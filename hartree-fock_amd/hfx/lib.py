@@ -102,7 +102,8 @@ class HfxScfResult(C.Structure):
 
 
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
-               "hfx_nuclear", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk", "hfx_jk_async", "hfx_last_error",
+               "hfx_nuclear", "hfx_one_electron_noalloc", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk",
+               "hfx_jk_async", "hfx_last_error",
                "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine", "hfx_scf_run_batched",
                "hfx_scf_run_batched_engine", "hfx_rhf_gradient", "hfx_store_eris", "hfx_release_eris"]
 
@@ -132,6 +133,8 @@ class Library:
         for f in ("hfx_overlap", "hfx_kinetic", "hfx_schwarz"):
             getattr(d, f).argtypes = [C.c_void_p, _f64p]
         d.hfx_nuclear.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _i32p, _f64p]
+        if hasattr(d, "hfx_one_electron_noalloc"):
+            d.hfx_one_electron_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p, _i32p, _f64p]
         d.hfx_eri_quartets.argtypes = [C.c_void_p, C.c_int64, _i32p, C.POINTER(C.c_int64), _f64p]
         d.hfx_jk.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.POINTER(HfxJkStats)]
         if hasattr(d, "hfx_jk_async"):  # (a library built before this entry point lacks it)

@@ -143,6 +143,18 @@ int hfx_kinetic(hfx_engine* e, double* T);
 int hfx_nuclear(hfx_engine* e, int32_t n_atoms, const double* Z, const double* xyz,
                 const int32_t* atom_mol, double* V);
 
+/* The one-electron matrices without any device allocation: kind 0 = hfx_overlap, 1 = hfx_kinetic,
+ * 2 = hfx_nuclear (same arguments and results as those calls; for kinds 0 and 1 the atoms are
+ * ignored).  No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The atoms
+ * and the output are staged in a static device array of 16.2 MiB that every GPU the library runs on
+ * holds until the process ends, so the engine's packed matrices may hold at most 2^21 elements
+ * (sum of N_m^2: one molecule of up to 1448 functions), at most 4096 atoms and 4096 molecules, with
+ * the atoms grouped by molecule in molecule order; larger inputs fail with HFX_ERR_INVALID.  Calls
+ * on one GPU share that array: calls that overlap in time on one GPU race (serialize them). */
+int hfx_one_electron_noalloc(hfx_engine* e, int32_t kind, int32_t n_atoms, const double* Z, const double* xyz,
+                             const int32_t* atom_mol, double* M);
+
 /* IntegralEngine::schwartzScreeningMatrix (src/IntegralEngine.cpp:695-728): nS x nS. */
 int hfx_schwarz(hfx_engine* e, double* Q);
 

@@ -1270,6 +1270,15 @@ int hfx_nuclear(hfx_engine* e, int32_t nat, const double* Z, const double* xyz, 
     return one_electron(e, 2, nat, Z, xyz, atom_mol, V);
 }
 
+// On the CPU there is no device memory to avoid: the same integrals as the allocating calls.
+int hfx_one_electron_noalloc(hfx_engine* e, int32_t kind, int32_t n_atoms, const double* Z, const double* xyz,
+                             const int32_t* atom_mol, double* M) {
+    if (kind == 0) return hfx_overlap(e, M);
+    if (kind == 1) return hfx_kinetic(e, M);
+    if (kind == 2) return hfx_nuclear(e, n_atoms, Z, xyz, atom_mol, M);
+    return fail(HFX_ERR_INVALID, "hfx_one_electron_noalloc: kind must be 0, 1 or 2");
+}
+
 int hfx_schwarz(hfx_engine* e, double* Q) {
     if (!e || !Q) return fail(HFX_ERR_INVALID, "null argument");
     int64_t off = 0;
