@@ -38,7 +38,7 @@ import numpy as np  # noqa: E402
 
 from hfx import Basis, autotune, device, geometry, lib  # noqa: E402
 
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X: FP64 vector / dense MFMA; FP32 dense MFMA
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X listed vector peaks (FP32 counts packed FP32)
 PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E
 CLASS_NAMES = ["ssss", "psss", "psps", "ppss", "ppps", "pppp", "dsss", "dsps", "dspp", "dsds", "dpss", "dpps",
                "dppp", "dpds", "dpdp", "ddss", "ddps", "ddpp", "ddds", "dddp", "dddd"]
@@ -58,14 +58,17 @@ def timed_builds(L, eng, bufs, n, steps, warmup, precision, cut, Q, flags=lib.HF
     device.synchronize()
     ms64 = np.zeros(21)
     ms32 = np.zeros(21)
+    dev_ms = []
     t0 = time.perf_counter()
     st = None
     for _ in range(steps):
         st = lib.jk_device(eng, dDt, dDa, J, Ka, **kw)
         ms64 += st["class_ms64"]
         ms32 += st["class_ms32"]
+        dev_ms.append(st["device_ms"])
     device.synchronize()
     dt = time.perf_counter() - t0
+    st = dict(st, device_ms_median=float(np.median(dev_ms)))
     return dt, st, ms64 / steps, ms32 / steps
 
 
@@ -107,7 +110,13 @@ def cpu_baseline(basis, Da, Q, total_gflop, budget_s):
     rate = g / t if t > 0 else 0.0  # model GFLOP/s on this slice
     builds = rate / total_gflop if total_gflop > 0 else 0.0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=builds, unit="builds/s", cores=threads, kind="port",
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return dict(value=builds, unit="builds/s", cores=threads, kind="port", cpu_model=model,
                 gflops_model=rate,
                 sample="bra shells %d..%d of %d (%.1f of %.1f model GFLOP) of the same FP64 build, %.1f s" %
                        (mid, min(nS, mid + width) - 1, nS, g, total_gflop, t))
@@ -170,6 +179,7 @@ def measured_traffic(group):
 def stored_run(L, basis, eng, bufs, Q, steps, warmup, J64, K64, Da, scf_kw):
     """The conventional path at the same density: store, K graph-replayed stored builds, per-class
     times, the HBM roofline of the dominant stored-digest group, and the conventional SCF."""
+    eng.set_launch(2, 8192, 4)  # the stored builds' defaults (longer claimed runs: fewer row flushes)
     t0 = time.perf_counter()
     nq, nbytes = eng.store_eris(1e-10, Q=Q)
     t_store = time.perf_counter() - t0
@@ -203,17 +213,26 @@ def stored_run(L, basis, eng, bufs, Q, steps, warmup, J64, K64, Da, scf_kw):
 
 
 def roofline_of(stmx, cm64, cm32):
-    """Roofline of the dominant class kernel group (one class, one precision) of a build."""
+    """Roofline of the dominant class kernel group (one class, one precision) of a build.
+
+    `achieved` counts the model FLOPs of the primitive quartets the kernel executes (the engine
+    prunes primitive pairs below 1e-20 / 1e-11 in FP64 / FP32, which the unpruned model count of
+    the survey would still credit); the kernels are VALU-bound (no MFMA), so the bound is the
+    vector pipe, priced at MI355X's listed vector peak (FP32 157.3 TFLOP/s counts packed FP32,
+    which these kernels do not use: their non-packed ceiling is 78.6, given as peak_nonpacked)."""
     groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
     ms, prec, k = max(groups)
-    gfl = (stmx["class_gflop64"] if prec == "f64" else stmx["class_gflop32"])[k]
+    gfl = (stmx["class_exec_gflop64"] if prec == "f64" else stmx["class_exec_gflop32"])[k]
+    gfl_model = (stmx["class_gflop64"] if prec == "f64" else stmx["class_gflop32"])[k]
     achieved = gfl / ms if ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
     group = "%s %s" % (CLASS_NAMES[k], prec)
-    return dict(bound="mfma", pipe="VALU %s (MI355X peak listed for %s)" % (prec, prec), kernel="k_eri " + group,
+    return dict(bound="valu", pipe="VALU %s" % prec, kernel="k_eri " + group,
                 achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s", frac=achieved / PEAK_TFLOPS[prec],
+                peak_nonpacked=78.6, frac_nonpacked=achieved / 78.6,
                 traffic=measured_traffic(group), traffic_source="rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch, "
-                "profiles/k_eri_c20h42_mixed_traffic.md", launch_ms=ms, launch_model_gflop=gfl,
-                build_achieved_tflops=stmx["model_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
+                "profiles/k_eri_c20h42_mixed_traffic.md", launch_ms=ms, launch_exec_gflop=gfl,
+                launch_model_gflop_unpruned=gfl_model,
+                build_exec_tflops=stmx["exec_gflop"] / stmx["device_ms"] if stmx["device_ms"] > 0 else 0.0)
 
 
 def profile_builds(args):
@@ -291,11 +310,18 @@ def main():
 
     eng = lib.Engine(L, basis)
     Q = eng.schwarz()
-    cut, table = autotune.tune_cut(eng, Da, target=1e-7, Q=Q)
+    # the cut: the fastest whose build-level |dE2| against FP64 stays within half the 1e-6 Eh budget
+    cut, table = autotune.tune_cut(eng, Da, target=5e-7, Q=Q)
     for row in table:
         log("cut %.0e: |dE2| %.2e, %.3fs, fp32 %d / %d" % row)
     if cut == 0.0:
         cut = 1e-6
+    # launch parameters (blocks per CU, claimed run length, streams), tuned for each precision
+    launch64, lt64 = autotune.tune_launch(eng, Da, Q=Q, precision=lib.HFX_FP64)
+    launchmx, ltmx = autotune.tune_launch(eng, Da, Q=Q, precision=lib.HFX_MIXED, cut=cut)
+    log("launch tuning FP64 %s, mixed %s" % (launch64, launchmx))
+    for (cfg, ms64_), (_, msmx_) in zip(lt64, ltmx):
+        log("  blocks/CU %d, chunk %d, streams %d: FP64 %.2f ms, mixed %.2f ms" % (cfg + (ms64_, msmx_)))
     rmx = lib.scf_run(L, basis, options=L.default_options(precision=lib.HFX_MIXED, fp32_cut=cut, final_fp64=1,
                                                           **scf_kw))
     dE_mixed = abs(rmx["e_total"] - r64["e_total"])
@@ -309,15 +335,19 @@ def main():
             device.DeviceArray(n * n))
     plain = lib.HFX_JK_NO_STATS
     graph = lib.HFX_JK_NO_STATS | lib.HFX_JK_USE_GRAPH
+    eng.set_launch(*launch64)
     t64e, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q, plain)
-    t64, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q, graph)
+    t64, sg64, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_FP64, 0.0, Q, graph)
     J64, K64 = bufs[2].download(), bufs[3].download()
+    eng.set_launch(*launchmx)
     tmxe, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q, plain)
-    tmx, _, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q, graph)
+    tmx, sgmx, _, _ = timed_builds(L, eng, bufs, n, args.steps, args.warmup, lib.HFX_MIXED, cut, Q, graph)
     Jmx, Kmx = bufs[2].download(), bufs[3].download()
     log("builds/s: FP64 eager %.3f graph %.3f | mixed eager %.3f graph %.3f" % (
         args.steps / t64e, args.steps / t64, args.steps / tmxe, args.steps / tmx))
+    eng.set_launch(*launch64)
     _, st64, c64, _ = timed_builds(L, eng, bufs, n, args.steps, 1, lib.HFX_FP64, 0.0, Q)
+    eng.set_launch(*launchmx)
     _, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, 1, lib.HFX_MIXED, cut, Q)
     dF_build = float(max(np.abs(Jmx - J64).max(), np.abs(Kmx - K64).max()))
     dE_build = abs(float(np.sum(Da.reshape(-1, order="F") * ((Jmx - Kmx) - (J64 - K64)))))
@@ -370,7 +400,12 @@ def main():
         "config": {"workload": "C%dH%d/cc-pVDZ expanded (%d Cartesian functions), full direct Fock build at the "
                                "converged FP64 density, Schwarz 1e-10, density 1e-10" % (
                                    args.carbons, 2 * args.carbons + 2, n),
-                   "fp32_cut": cut, "quartets": st64["n_quartets"], "model_gflop": st64["model_gflop"]},
+                   "fp32_cut": cut, "quartets": st64["n_quartets"], "model_gflop": st64["model_gflop"],
+                   "exec_gflop_fp64": st64["exec_gflop"], "exec_gflop_mixed": stmx["exec_gflop"],
+                   "launch_fp64": dict(zip(("blocks_per_cu", "chunk_len", "streams"), launch64)),
+                   "launch_mixed": dict(zip(("blocks_per_cu", "chunk_len", "streams"), launchmx))},
+        "ms_per_build_event_median_mixed": sgmx["device_ms_median"],
+        "ms_per_build_event_median_fp64": sg64["device_ms_median"],
         "fp64_builds_per_s": b64,
         "fp64_eager_builds_per_s": args.steps / t64e,
         "mixed_eager_builds_per_s": args.steps / tmxe,

@@ -699,7 +699,9 @@ struct hfx_engine {
     // Streams the class kernels of a build are spread over (side[0] is `stream`); the
     // captured graph has one branch per stream.
     static constexpr int N_SIDE = 4;
-    int n_side = N_SIDE;  // streams in use (HFX_BUILD_STREAMS, 1..N_SIDE)
+    int n_side = N_SIDE;  // streams in use (HFX_BUILD_STREAMS, 1..N_SIDE, or hfx_set_launch)
+    int blocks_per_cu = 2;  // persistent blocks per CU of a class launch (HFX_BLOCKS_PER_CU, hfx_set_launch)
+    int chunk_len = 0;      // list entries a block claims at a time (0: the defaults below; HFX_CHUNK_LEN)
     hipStream_t side[N_SIDE] = {};
     hipEvent_t ev_fork = nullptr, ev_join[N_SIDE] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1008,13 +1010,7 @@ EriArgs eri_args(hfx_engine* e) {
 
 // Blocks per class launch: two per CU (HFX_BLOCKS_PER_CU to change).  The launches of a build
 // run four at a time on the streams, and small persistent grids let them share the machine.
-int eri_grid(hfx_engine* e) {
-    static const int mult = [] {
-        const char* v = std::getenv("HFX_BLOCKS_PER_CU");
-        return v ? std::max(1, std::min(64, std::atoi(v))) : 2;
-    }();
-    return e->n_cu * mult;
-}
+int eri_grid(hfx_engine* e) { return e->n_cu * e->blocks_per_cu; }
 
 // Store-mode ERI blocks for a list of (bra pair, ket pair) entries of one class.
 int store_blocks(hfx_engine* e, int combo, const std::vector<uint2>& ents, std::vector<double>& out,
@@ -1111,6 +1107,8 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     }
     HIPCHK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
     if (const char* v = std::getenv("HFX_BUILD_STREAMS")) e->n_side = std::max(1, std::min(hfx_engine::N_SIDE, std::atoi(v)));
+    if (const char* v = std::getenv("HFX_BLOCKS_PER_CU")) e->blocks_per_cu = std::max(1, std::min(16, std::atoi(v)));
+    if (const char* v = std::getenv("HFX_CHUNK_LEN")) e->chunk_len = std::max(256, std::min(65536, std::atoi(v)));
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
     HIPCHK(hipEventCreateWithFlags(&e->ev_async, hipEventDisableTiming));
@@ -1515,11 +1513,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     hipLaunchKernelGGL(k_fill_u32, dim3(4), dim3(256), 0, s, e->d_chunk_ctr, 2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
     // Longer runs flush a stored build's exchange rows less often; in the direct build they leave
     // a long tail on the heaviest-first lists (ssss FP32 2.9 -> 7.5 ms alone at 8192).
-    static const int chunk_env = [] {
-        const char* v = std::getenv("HFX_CHUNK_LEN");
-        return v ? std::max(256, std::min(1 << 20, std::atoi(v))) : 0;
-    }();
-    ea.chunk_len = chunk_env ? chunk_env : (P.stored ? hfx_engine::STORED_CHUNK_LEN : hfx_engine::CHUNK_LEN);
+    ea.chunk_len = e->chunk_len ? e->chunk_len : (P.stored ? hfx_engine::STORED_CHUNK_LEN : hfx_engine::CHUNK_LEN);
     // LDS exchange rows of a class kernel: NA + NB rows of max_N doubles (twice for UHF), next
     // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without.
     auto tile_cols = [&](int k, int fp32) {
@@ -2351,6 +2345,18 @@ int hfx_eri_quartets(hfx_engine* e, int64_t n, const int32_t* abcd, const int64_
 
 int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, hfx_jk_stats* st) {
     return jk_impl(e, args, J, Ka, Kb, st);
+}
+
+int hfx_set_launch(hfx_engine* e, int32_t blocks_per_cu, int32_t chunk_len, int32_t streams) {
+    if (!e) return fail(HFX_ERR_INVALID, "hfx_set_launch: null engine");
+    if (blocks_per_cu < 0 || blocks_per_cu > 16 || (chunk_len != 0 && (chunk_len < 256 || chunk_len > 65536)) ||
+        streams < 0 || streams > hfx_engine::N_SIDE)
+        return fail(HFX_ERR_INVALID, "hfx_set_launch: blocks_per_cu 1..16, chunk_len 256..65536, streams 1..4 (0 keeps)");
+    if (blocks_per_cu) e->blocks_per_cu = blocks_per_cu;
+    if (chunk_len) e->chunk_len = chunk_len;
+    if (streams) e->n_side = streams;
+    e->drop_graph();  // the captured build has the old launch shapes
+    return HFX_OK;
 }
 
 int hfx_jk_async(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, void* stream) {

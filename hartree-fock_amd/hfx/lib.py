@@ -103,7 +103,7 @@ class HfxScfResult(C.Structure):
 
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
                "hfx_nuclear", "hfx_one_electron_noalloc", "hfx_schwarz", "hfx_eri_quartets", "hfx_jk",
-               "hfx_jk_async", "hfx_last_error",
+               "hfx_jk_async", "hfx_set_launch", "hfx_last_error",
                "hfx_scf_default_options", "hfx_scf_run", "hfx_scf_run_engine", "hfx_scf_run_batched",
                "hfx_scf_run_batched_engine", "hfx_rhf_gradient", "hfx_store_eris", "hfx_release_eris"]
 
@@ -137,6 +137,8 @@ class Library:
             d.hfx_one_electron_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p, _i32p, _f64p]
         d.hfx_eri_quartets.argtypes = [C.c_void_p, C.c_int64, _i32p, C.POINTER(C.c_int64), _f64p]
         d.hfx_jk.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.POINTER(HfxJkStats)]
+        if hasattr(d, "hfx_set_launch"):
+            d.hfx_set_launch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
         if hasattr(d, "hfx_jk_async"):  # (a library built before this entry point lacks it)
             d.hfx_jk_async.argtypes = [C.c_void_p, C.POINTER(HfxJkArgs), _f64p, _f64p, _f64p, C.c_void_p]
         d.hfx_store_eris.argtypes = [C.c_void_p, _f64p, C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -287,6 +289,10 @@ class Engine:
         self.lib.check(self.lib.dll.hfx_store_eris(self.h, _p(q), schwarz_thresh, C.byref(n), C.byref(b)),
                        "hfx_store_eris")
         return n.value, b.value
+
+    def set_launch(self, blocks_per_cu=0, chunk_len=0, streams=0):
+        """hfx_set_launch: class-kernel launch parameters of this engine (0 keeps a value)."""
+        self.lib.check(self.lib.dll.hfx_set_launch(self.h, blocks_per_cu, chunk_len, streams), "hfx_set_launch")
 
     def release_eris(self):
         self.lib.check(self.lib.dll.hfx_release_eris(self.h), "hfx_release_eris")

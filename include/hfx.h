@@ -192,6 +192,14 @@ int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double
  *     replayed while another call on the same engine runs. */
 int hfx_jk_async(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double* Kb, void* stream);
 
+/* Launch parameters of this engine's class kernels, the autotuner's knobs (0 keeps a value):
+ * blocks_per_cu = persistent blocks of 256 threads per CU in each class launch (1..16, default 2),
+ * chunk_len = longest run of list entries a block claims at a time (256..65536; default 2048, 8192 in
+ * stored builds), streams = streams a build's class kernels are spread over (1..4, default 4).  From
+ * the next build on (a captured build is captured again).  No reference counterpart: the reference's
+ * quartet loop has the fixed OpenMP schedule(dynamic,32) (src/SCF.cpp:592). */
+int hfx_set_launch(hfx_engine* e, int32_t blocks_per_cu, int32_t chunk_len, int32_t streams);
+
 /* Conventional (stored-integral) mode.  Computes once, and keeps in HBM, the normalized
  * integral blocks of every canonical shell quartet with Q(a,b)Q(c,d) >= schwarz_thresh (Q as
  * in hfx_jk_args: the caller's table or, when NULL, the engine's own).  Replaces

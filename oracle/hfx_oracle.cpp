@@ -1347,6 +1347,14 @@ int hfx_jk(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka, double
     return jk_impl(e, args, J, Ka, Kb, st, 0, 1 << 30);
 }
 
+// Launch parameters of device kernels: nothing to set on the CPU (checked like the engine does).
+int hfx_set_launch(hfx_engine* e, int32_t blocks_per_cu, int32_t chunk_len, int32_t streams) {
+    if (!e || blocks_per_cu < 0 || blocks_per_cu > 16 || (chunk_len != 0 && (chunk_len < 256 || chunk_len > 65536)) ||
+        streams < 0 || streams > 4)
+        return fail(HFX_ERR_INVALID, "hfx_set_launch: bad arguments");
+    return HFX_OK;
+}
+
 // The CPU oracle has no device and no stream: the "device" buffers of the async variant are host
 // buffers and the build completes before the call returns (HFX_JK_DETERMINISTIC is accepted and
 // changes nothing here).
