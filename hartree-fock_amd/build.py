@@ -3,6 +3,7 @@
 Translation units:
   csrc/engine.hip             host engine, screening / one-electron / symmetrization kernels
   csrc/k_grad.hip             RHF nuclear-gradient kernels
+  csrc/hfx_scf_gpu.hip        the SCF iteration on the device (rocBLAS/rocSOLVER loaded on first use)
   csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
                               are fully unrolled, so they are compiled in parallel slices
   csrc/hfx_scf.cpp            host SCF driver (g++ with OpenMP)
@@ -68,6 +69,7 @@ def jobs():
     srcs = []
     srcs.append(("engine", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "engine.hip")]))
     srcs.append(("k_grad", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "k_grad.hip")]))
+    srcs.append(("hfx_scf_gpu", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "hfx_scf_gpu.hip")]))
     for (k, la, lb, lc, ld, nch) in combo_classes():
         step = CHUNKS_PER_TU_HEAVY if la + lb + lc + ld >= 6 else CHUNKS_PER_TU
         for b in range(0, nch, step):
@@ -120,7 +122,7 @@ def build(jobs_n=None, verbose=False, only=None):
         raise RuntimeError("HIP build failed: " + ", ".join(n for n, _ in failed))
     objs = sorted(objs + keep)
     link = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + \
-        ["-lgomp", "-Wl,-Bsymbolic", "-Wl,--no-undefined"]
+        ["-lgomp", "-ldl", "-Wl,-Bsymbolic", "-Wl,--no-undefined"]
     r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])

@@ -38,6 +38,7 @@
 
 #include "engine.hpp"
 #include "md.hpp"
+#include "scf_device.hpp"
 
 using namespace hfx;
 
@@ -2089,6 +2090,18 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
 }
 
 }  // namespace
+
+namespace hfx_internal {
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int engine_device(const hfx_engine* e, int* device) {
+    if (!e || !device) return fail(HFX_ERR_INVALID, "null engine");
+    *device = e->device;
+    return HFX_OK;
+}
+
+}  // namespace hfx_internal
 
 extern "C" {
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "linalg.hpp"
+#include "scf_device.hpp"
 
 namespace {
 
@@ -367,6 +368,7 @@ struct Scf {
         res->time_diag_s = t_diag;
         res->n_jk_builds = n_builds;
         res->n_quartets_last = n_quart_last;
+        res->time_linalg_load_s = 0.0;
         auto put = [&](double* dst, const Mat& src) {
             if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
         };
@@ -404,8 +406,6 @@ struct Scf {
         return HFX_OK;
     }
 };
-
-thread_local std::string g_scf_err;
 
 // Conventional mode (DIRECT false, src/SCF.cpp:57-60, 159): every build from the full density
 // through the integrals stored once per run; they are FP64, so the builds are too.
@@ -491,6 +491,15 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
     }
     s.h = s.mat();
     for (size_t x = 0; x < s.h.size(); ++x) s.h[x] = T[x] + V[x];
+    if (s.o.device_linalg) {  // the cycles with the iteration's linear algebra on the device
+        const hfx_internal::ScfSetup su{s.n, s.nS, s.na, s.nb, s.uhf, s.precision, s.e_nuc, s.S.data(), s.h.data(),
+                                        s.Q.data()};
+        const double t_setup = secs(t0, clk::now());
+        if ((rc = hfx_internal::device_cycles(e, s.o, su, res))) return rc;
+        res->time_init_s += t_setup;
+        res->time_total_s = secs(t0, clk::now());
+        return HFX_OK;
+    }
     s.X = inv_sqrt(s.n, s.S);
     s.initial_guess();
     const auto t_init = clk::now();
@@ -563,6 +572,8 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
         o = *opt;
     else
         hfx_scf_default_options(&o);
+    if (o.device_linalg)
+        return hfx_internal::set_error(HFX_ERR_UNSUPPORTED, "hfx_scf_run_batched: device_linalg runs one molecule");
     conventional_options(o);
 
     // Per-molecule atoms (contiguous runs of atom_mol) and packed block offsets.

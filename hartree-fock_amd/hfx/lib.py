@@ -86,7 +86,7 @@ class HfxScfOptions(C.Structure):
                 ("max_lshift_iter", C.c_int32), ("stop_lshift_thresh", C.c_double), ("precision", C.c_int32),
                 ("fp32_cut", C.c_double), ("final_fp64", C.c_int32), ("jk_flags", C.c_int32),
                 ("full_builds", C.c_int32), ("direct", C.c_int32),
-                ("switch_tol", C.c_double)]
+                ("switch_tol", C.c_double), ("device_linalg", C.c_int32)]
 
 
 class HfxScfResult(C.Structure):
@@ -98,7 +98,8 @@ class HfxScfResult(C.Structure):
                 ("n_quartets_last", C.c_int64),
                 ("F_alpha", _f64p), ("D_alpha", _f64p), ("C_alpha", _f64p), ("eps_alpha", _f64p),
                 ("F_beta", _f64p), ("D_beta", _f64p), ("C_beta", _f64p), ("eps_beta", _f64p),
-                ("h_core", _f64p), ("S", _f64p), ("trace", _f64p), ("trace_cap", C.c_int32)]
+                ("h_core", _f64p), ("S", _f64p), ("trace", _f64p), ("trace_cap", C.c_int32),
+                ("time_linalg_load_s", C.c_double)]
 
 
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
@@ -376,8 +377,10 @@ def jk_async(engine, dD_tot, dD_alpha, J, Ka, stream, dD_beta=None, Kb=None, Q=N
     engine.lib.check(engine.lib.dll.hfx_jk_async(engine.h, C.byref(a), vp(J), vp(Ka), vp(Kb), h), "hfx_jk_async")
 
 
-def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, outputs=True, trace_cap=200):
-    """Run the library's SCF driver; returns a dict with energies and (optionally) matrices."""
+def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, outputs=True, trace_cap=200,
+            engine=None):
+    """Run the library's SCF driver (hfx_scf_run, or hfx_scf_run_engine over `engine`); returns a
+    dict with energies and (optionally) matrices."""
     o = options if options is not None else library.default_options()
     r = HfxScfResult()
     n = basis.n_aos
@@ -395,9 +398,14 @@ def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, ou
     hb = _BasisHolder(basis)
     Z = np.ascontiguousarray(np.asarray(basis.Z, np.float64))
     xyz = np.ascontiguousarray(basis.xyz.reshape(-1))
-    rc = library.dll.hfx_scf_run(C.byref(hb.struct), len(Z), _p(Z), _p(xyz), charge, multiplicity, C.byref(o),
-                                 device, C.byref(r))
-    library.check(rc, "hfx_scf_run")
+    if engine is None:
+        rc = library.dll.hfx_scf_run(C.byref(hb.struct), len(Z), _p(Z), _p(xyz), charge, multiplicity, C.byref(o),
+                                     device, C.byref(r))
+        library.check(rc, "hfx_scf_run")
+    else:
+        rc = library.dll.hfx_scf_run_engine(engine.h, len(Z), _p(Z), _p(xyz), charge, multiplicity, C.byref(o),
+                                            C.byref(r))
+        library.check(rc, "hfx_scf_run_engine")
     out = {k: getattr(r, k) for k, _ in HfxScfResult._fields_ if not k.startswith(("F_", "D_", "C_", "eps_", "h_", "S", "trace"))}
     it = r.iterations
     out["trace"] = keep["trace"][:4 * min(it, trace_cap)].reshape(-1, 4)

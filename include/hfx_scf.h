@@ -50,6 +50,12 @@ typedef struct hfx_scf_options {
                                /* below this, then continue with FP64 builds (the first one   */
                                /* full) keeping the DIIS history; 0: converge in mixed        */
                                /* precision first, then restart FP64 with a fresh DIIS        */
+    int32_t device_linalg;     /* 1: the iteration's linear algebra (DIIS, X^T F X, the       */
+                               /* eigensolver, densities, energy) on the engine's device      */
+                               /* (rocBLAS/rocSOLVER, loaded on first use) with the builds    */
+                               /* stream-ordered through hfx_jk_async; the host reads a few   */
+                               /* scalars per iteration.  Single-molecule runs; diis_size     */
+                               /* 1..15.  0 (default): the host driver's own linear algebra   */
 } hfx_scf_options;
 
 typedef struct hfx_scf_result {
@@ -75,6 +81,9 @@ typedef struct hfx_scf_result {
     /* Optional per-iteration trace: trace[4*it + {0,1,2,3}] = E_elec, dE, dD, DIIS error. */
     double* trace;
     int32_t trace_cap;           /* number of iterations the trace can hold                 */
+    double time_linalg_load_s;   /* device_linalg: seconds spent loading rocBLAS/rocSOLVER   */
+                                 /* and creating the device's handle (first run in a process */
+                                 /* only; inside time_init_s and time_total_s)               */
 } hfx_scf_result;
 
 void hfx_scf_default_options(hfx_scf_options* opt);

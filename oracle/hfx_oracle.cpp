@@ -22,6 +22,7 @@
 // -ffp-contract=fast; this file is built with -ffp-contract=off.
 
 #include "../include/hfx.h"
+#include "../hartree-fock_amd/csrc/scf_device.hpp"
 
 #include <algorithm>
 #include <array>
@@ -1193,6 +1194,18 @@ void stored_jk(const Mol& m, const double* Dt, const double* Da, const double* D
 }
 
 }  // namespace
+
+// The host SCF driver linked into this library reports through this library's error text; the
+// SCF iteration with device linear algebra needs the HIP library and is refused here.
+namespace hfx_internal {
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int device_cycles(hfx_engine*, const hfx_scf_options&, const ScfSetup&, hfx_scf_result*) {
+    return fail(HFX_ERR_UNSUPPORTED, "device_linalg needs the HIP library");
+}
+
+}  // namespace hfx_internal
 
 extern "C" {
 

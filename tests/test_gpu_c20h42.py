@@ -104,3 +104,20 @@ def test_c20h42_scf_fp64_and_mixed(engine_lib, c20):
     print("c20h42 scf: fp64 %.10f (%d it), mixed %.10f (%d+%d it), oracle E(D) %.10f" % (
         r64["e_total"], r64["iterations"], rmx["e_total"], rmx["iterations_mixed"],
         rmx["iterations"] - rmx["iterations_mixed"], e_ref), flush=True)
+
+
+def test_c20h42_scf_device_linalg(engine_lib, c20):
+    """The mixed-precision SCF with the iteration's linear algebra on the device (rocSOLVER's
+    eigensolver, DIIS and densities in HBM) over the benchmark's engine: the oracle's energy at
+    the converged density to 1e-6 Eh, as the host-driven run."""
+    b, D, Q, ref, g = c20
+    print("c20h42 device scf", flush=True)
+    e_ref = ref["e_electronic"] + ref["e_nuclear"]
+    kw = dict(precision=lib.HFX_MIXED, fp32_cut=1e-2, final_fp64=1, jk_flags=lib.HFX_JK_USE_GRAPH,
+              device_linalg=1, **SCF)
+    r = lib.scf_run(engine_lib, b, options=engine_lib.default_options(**kw), outputs=False, engine=g)
+    assert r["converged"] == 1 and 0 < r["iterations_mixed"] < r["iterations"]
+    assert abs(r["e_total"] - e_ref) <= 1e-6
+    print("c20h42 device scf: %.10f (%d+%d it) in %.3f s (J/K %.3f s, rest %.3f s, library load %.3f s)" % (
+        r["e_total"], r["iterations_mixed"], r["iterations"] - r["iterations_mixed"], r["time_total_s"],
+        r["time_jk_s"], r["time_diag_s"], r["time_linalg_load_s"]), flush=True)

@@ -393,6 +393,14 @@ def test_screening_counts_match_bruteforce(oracle):
 # ---------------------------------------------------------------------------------------
 # SCF driver against an independent numpy SCF on the oracle's integrals
 # ---------------------------------------------------------------------------------------
+def test_device_linalg_needs_the_hip_library(oracle):
+    """The host driver linked into the oracle refuses the device iteration with a clear error."""
+    b = molecules.h2o("sto-3g")
+    with pytest.raises(lib.HfxError, match="HIP library"):
+        lib.scf_run(oracle, b, options=oracle.default_options(device_linalg=1))
+    assert lib.scf_run(oracle, b)["converged"] == 1
+
+
 def test_scf_against_numpy(oracle):
     import scipy.linalg as sl
     b = molecules.h2o("6-31g**")
