@@ -243,26 +243,6 @@ def profile_builds(args):
     Da = np.load(args.density)
     eng = lib.Engine(L, basis)
     Q = eng.schwarz()
-    # the same two SCFs with the iteration's linear algebra on the device (rocSOLVER eigensolver, DIIS
-    # and densities in HBM); the first such run in the process loads rocBLAS/rocSOLVER (reported apart)
-    r64d = lib.scf_run(L, basis, options=L.default_options(device_linalg=1, **scf_kw), outputs=False)
-    rmxd = lib.scf_run(L, basis, options=L.default_options(device_linalg=1, precision=lib.HFX_MIXED, fp32_cut=cut,
-                                                           final_fp64=1, jk_flags=lib.HFX_JK_USE_GRAPH, **scf_kw),
-                       outputs=False)
-    scf_dev = dict(wall_s_fp64=r64d["time_total_s"] - r64d["time_linalg_load_s"],
-                   wall_s_mixed=rmxd["time_total_s"] - rmxd["time_linalg_load_s"],
-                   linalg_load_s=r64d["time_linalg_load_s"] + rmxd["time_linalg_load_s"],
-                   jk_s_mixed=rmxd["time_jk_s"], rest_s_mixed=rmxd["time_diag_s"], init_s_mixed=rmxd["time_init_s"],
-                   iterations_fp64=r64d["iterations"],
-                   iterations_mixed=[rmxd["iterations_mixed"], rmxd["iterations"] - rmxd["iterations_mixed"]],
-                   dE_fp64_vs_host=abs(r64d["e_total"] - r64["e_total"]),
-                   dE_mixed_vs_fp64=abs(rmxd["e_total"] - r64["e_total"]),
-                   converged=[r64d["converged"], rmxd["converged"]])
-    log("device-linalg SCF: FP64 %.2fs (%d it), mixed %.2fs (%d+%d it; J/K %.2fs, rest %.2fs), library load %.2fs, "
-        "|dE| mixed vs FP64 %.2e" % (scf_dev["wall_s_fp64"], r64d["iterations"], scf_dev["wall_s_mixed"],
-                                      *scf_dev["iterations_mixed"], rmxd["time_jk_s"], rmxd["time_diag_s"],
-                                      scf_dev["linalg_load_s"], scf_dev["dE_mixed_vs_fp64"]))
-
     bufs = (device.DeviceArray.from_host(2 * Da), device.DeviceArray.from_host(Da), device.DeviceArray(n * n),
             device.DeviceArray(n * n))
     # A plain device-to-device copy of known size, so that a counter run can calibrate its

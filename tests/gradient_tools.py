@@ -86,3 +86,23 @@ def scf_energy_function(library, basis_name, Z, **kw):
             return grad_out * g
 
     return ScfEnergy
+
+
+def autograd_check():
+    """Backward of the torch function against central finite differences on four coordinates
+    of the water dimer (6-31G**); prints one JSON line.  Imports torch before the engine."""
+    import json
+
+    import torch
+
+    import molecules
+    b = molecules.water_dimer("6-31g**")
+    library = lib.load()
+    kw = dict(full_builds=0, energy_tol=1e-10, density_tol=1e-8, schwarz_thresh=1e-12, density_thresh=1e-12)
+    F = scf_energy_function(library, b.name, b.Z, **kw)
+    x = torch.tensor(b.xyz, dtype=torch.float64, requires_grad=True)
+    E = F.apply(x)
+    E.backward()
+    fd = finite_differences(library, b, coords=[(0, 0), (1, 1), (3, 2), (5, 0)], **kw)
+    err = max(abs(x.grad[A, c].item() - v) for (A, c), v in fd.items())
+    print(json.dumps({"n": len(fd), "max_err": err}), flush=True)

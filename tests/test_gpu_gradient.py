@@ -88,16 +88,20 @@ def test_gradient_mixed_precision(engine_lib):
     assert np.abs(gmx - g64).max() <= 1e-6
 
 
-def test_autograd_function(engine_lib):
+def test_autograd_function():
     """The SCF energy as a torch function of the coordinates: backward gives the analytic
-    gradient, which matches finite differences."""
-    torch = pytest.importorskip("torch")
-    b = molecules.water_dimer("6-31g**")
-    kw = dict(full_builds=0, energy_tol=1e-10, density_tol=1e-8, schwarz_thresh=1e-12, density_thresh=1e-12)
-    F = gt.scf_energy_function(engine_lib, b.name, b.Z, **kw)
-    x = torch.tensor(b.xyz, dtype=torch.float64, requires_grad=True)
-    E = F.apply(x)
-    E.backward()
-    fd = gt.finite_differences(engine_lib, b, coords=[(0, 0), (1, 1), (3, 2), (5, 0)], **kw)
-    for (A, c), v in fd.items():
-        assert abs(x.grad[A, c].item() - v) <= 1e-6
+    gradient, which matches finite differences.  Runs in a fresh process: PyTorch's ROCm
+    libraries cannot be loaded into a process that already holds the system rocBLAS/rocSOLVER
+    (the device-linear-algebra SCF tests load them), so torch is imported there first."""
+    pytest.importorskip("torch")
+    import subprocess
+    import sys
+    env = dict(os.environ)
+    env["PYTHONPATH"] = os.pathsep.join([HERE, os.path.join(os.path.dirname(HERE), "hartree-fock_amd")] +
+                                        ([env["PYTHONPATH"]] if env.get("PYTHONPATH") else []))
+    print("autograd function (child process)", flush=True)
+    p = subprocess.run([sys.executable, "-c", "import torch, gradient_tools as gt; gt.autograd_check()"], env=env,
+                       capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    assert res["n"] == 4 and res["max_err"] <= 1e-6, res
