@@ -2,7 +2,8 @@
 
 Translation units:
   csrc/engine.hip             host engine, screening / one-electron / symmetrization kernels
-  csrc/k_grad.hip             RHF nuclear-gradient kernels
+  csrc/k_grad.hip             RHF nuclear-gradient kernels (generic two-electron, one-electron, reduction)
+  csrc/k_grad_eri.hip (many)  one object per light quartet class: the unrolled two-electron gradient
   csrc/hfx_scf_gpu.hip        the SCF iteration on the device (rocBLAS/rocSOLVER loaded on first use)
   csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
                               are fully unrolled, so they are compiled in parallel slices
@@ -33,8 +34,10 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-I" + INC,
 CXXFLAGS = ["-O3", "-std=c++20", "-fPIC", "-fopenmp", "-I" + INC, "-Wall"]
 
 ACC_MAX = 36          # must match k_eri.hip
-CHUNKS_PER_TU = 6     # chunks per translation unit (one launch each; FP64, FP32 and stored-digest kernels)
-CHUNKS_PER_TU_HEAVY = 3  # the same for the classes with la+lb+lc+ld >= 6, whose units compile longest
+CHUNKS_PER_TU = 4     # chunks per translation unit (one launch each; FP64, FP32 and stored-digest kernels,
+                      # each with FP64-atomic and fixed-point accumulation)
+GRAD_LMAX_UNROLLED = 3  # gradient classes compiled unrolled (k_grad_eri.hip), one unit each
+CHUNKS_PER_TU_HEAVY = 2  # the same for the classes with la+lb+lc+ld >= 6, whose units compile longest
 
 
 def ncart(l):
@@ -70,6 +73,10 @@ def jobs():
     srcs.append(("engine", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "engine.hip")]))
     srcs.append(("k_grad", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "k_grad.hip")]))
     srcs.append(("hfx_scf_gpu", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "hfx_scf_gpu.hip")]))
+    for (k, la, lb, lc, ld, _) in combo_classes():
+        if la + lb + lc + ld <= GRAD_LMAX_UNROLLED:
+            srcs.append(("k_grad_eri_%02d" % k, [HIPCC] + HIPFLAGS + ["-DHFX_GRAD_COMBO=%d" % k, "-c",
+                                                                      os.path.join(CSRC, "k_grad_eri.hip")]))
     for (k, la, lb, lc, ld, nch) in combo_classes():
         step = CHUNKS_PER_TU_HEAVY if la + lb + lc + ld >= 6 else CHUNKS_PER_TU
         for b in range(0, nch, step):

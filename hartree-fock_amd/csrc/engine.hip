@@ -703,6 +703,7 @@ struct hfx_engine {
     int n_side = N_SIDE;  // streams in use (HFX_BUILD_STREAMS, 1..N_SIDE, or hfx_set_launch)
     int blocks_per_cu = 2;  // persistent blocks per CU of a class launch (HFX_BLOCKS_PER_CU, hfx_set_launch)
     int chunk_len = 0;      // list entries a block claims at a time (0: the defaults below; HFX_CHUNK_LEN)
+    int stagger = 0;        // staggered walk of the claimed runs (HFX_STAGGER)
     hipStream_t side[N_SIDE] = {};
     hipEvent_t ev_fork = nullptr, ev_join[N_SIDE] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1110,6 +1111,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
     if (const char* v = std::getenv("HFX_BUILD_STREAMS")) e->n_side = std::max(1, std::min(hfx_engine::N_SIDE, std::atoi(v)));
     if (const char* v = std::getenv("HFX_BLOCKS_PER_CU")) e->blocks_per_cu = std::max(1, std::min(16, std::atoi(v)));
     if (const char* v = std::getenv("HFX_CHUNK_LEN")) e->chunk_len = std::max(256, std::min(65536, std::atoi(v)));
+    if (const char* v = std::getenv("HFX_STAGGER")) e->stagger = std::atoi(v) ? 1 : 0;
     HIPCHK(hipEventCreate(&e->ev0));
     HIPCHK(hipEventCreate(&e->ev1));
     HIPCHK(hipEventCreateWithFlags(&e->ev_async, hipEventDisableTiming));
@@ -1515,6 +1517,7 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
     // Longer runs flush a stored build's exchange rows less often; in the direct build they leave
     // a long tail on the heaviest-first lists (ssss FP32 2.9 -> 7.5 ms alone at 8192).
     ea.chunk_len = e->chunk_len ? e->chunk_len : (P.stored ? hfx_engine::STORED_CHUNK_LEN : hfx_engine::CHUNK_LEN);
+    ea.stagger = e->stagger;
     // LDS exchange rows of a class kernel: NA + NB rows of max_N doubles (twice for UHF), next
     // to the kernel's Boys slice, within 64 KB per block; classes that do not fit run without.
     auto tile_cols = [&](int k, int fp32) {

@@ -92,6 +92,9 @@ struct EriArgs {
     // per row; 0: no LDS rows).
     unsigned int* chunk_ctr;
     int chunk_len;     // largest chunk; smaller lists use shorter chunks
+    int stagger;       // 1: each block walks its claimed run from a block-dependent starting
+                       // iteration (wrapping around), so that concurrent blocks of one class
+                       // reach the same ket pairs at different times
     int tile_n;
     int tile_shells;   // bits of the touched-ket-shell mask kept with the rows
     const int* sh_local;  // index of a shell inside its molecule
@@ -138,6 +141,11 @@ struct GradArgs {
     int n3;                  // 3 * n_atoms
 };
 void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, hipStream_t s);
+// The two-electron gradient kernels of one quartet class, compiled with every angular loop
+// unrolled (k_grad_eri.hip, one translation unit per class); classes without one use the
+// generic kernel of k_grad.hip.
+using GradClassFn = void (*)(int fp32, const GradArgs& g, int grid, hipStream_t s);
+bool register_grad_class(int combo, GradClassFn fn);
 void launch_grad_one(const GradArgs& g, int grid, hipStream_t s);
 void launch_grad_reduce(const GradArgs& g, int n_rows, double* out, hipStream_t s);
 

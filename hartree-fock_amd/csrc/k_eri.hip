@@ -98,10 +98,14 @@ __device__ __forceinline__ void fixed_add(unsigned long long* q, double v) {
     if (hi) __hip_atomic_fetch_add(q, (unsigned long long)hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (lo) __hip_atomic_fetch_add(q + 1, (unsigned long long)lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Element t of an accumulator block: a double (fx = 0) or a fixed-point pair at [2t, 2t+1] (fx = 1).
-__device__ __forceinline__ void acc_add(double* base, long long t, double v, int fx) {
+// Element t of an accumulator block: a double (FX false) or a fixed-point pair at [2t, 2t+1]
+// (FX true).  The accumulation mode is a kernel template parameter, not a run-time flag: the
+// fixed-point path's temporaries would otherwise count in every kernel's register allocation
+// at the digest's peak (dsps: one wavefront per SIMD instead of two).
+template <bool FX>
+__device__ __forceinline__ void acc_add(double* base, long long t, double v) {
     if (v == 0.0) return;
-    if (!fx)
+    if constexpr (!FX)
         __hip_atomic_fetch_add(base + t, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
         fixed_add(reinterpret_cast<unsigned long long*>(base) + 2 * t, v);
@@ -138,7 +142,7 @@ __device__ __forceinline__ long long stored_at(long long r, int e) {
 // holding bra pair tile_pid; NULL when the block keeps no rows.
 // STORED: the integrals are read from a.stored (normalized, written by the store mode) instead
 // of being computed; no Boys table.
-template <class R, int C0, bool STORED>
+template <class R, int C0, bool STORED, bool FX>
 __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool valid, const R* tab, double* tile,
                                         int tile_pid) {
     constexpr int CB = Chunks<R>::cb;
@@ -353,7 +357,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
             return (fabs(v) < tauS) ? 0.0 : v * s;
         };
         const long long copy = (long long)(xcd_id() % a.n_copies) * a.acc_stride;
-        const int fx = a.fixed;
+        constexpr int fx = FX ? 1 : 0;
         double* AJ = a.AJ + copy + (base << fx);
         {
             double Jab[CB], Jcd[NCD];
@@ -399,12 +403,12 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                         const double o = __shfl_down(x, off);
                         if (same[decltype(k_)::value]) x += o;
                     });
-                    if (head && valid) acc_add(AJ, tri_index(oA + ab / NB, oB + ab % NB), 2.0 * x, fx);
+                    if (head && valid) acc_add<FX>(AJ, tri_index(oA + ab / NB, oB + ab % NB), 2.0 * x);
                 }
             });
             sfor<NCD>([&](auto cd_) {
                 constexpr int cd = decltype(cd_)::value;
-                acc_add(AJ, tri_index(oC + cd / ND, oD + cd % ND), 2.0 * Jcd[cd], fx);
+                acc_add<FX>(AJ, tri_index(oC + cd / ND, oD + cd % ND), 2.0 * Jcd[cd]);
             });
         }
         // Exchange with one spin density Dx into the accumulator AKx (rows of a and b into the
@@ -459,11 +463,11 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                     } else {
                         sfor<NC>([&](auto jc_) {
                             constexpr int jc = decltype(jc_)::value;
-                            acc_add(AKx, tri_index(orow + r, oC + jc), Kc[r][jc], fx);
+                            acc_add<FX>(AKx, tri_index(orow + r, oC + jc), Kc[r][jc]);
                         });
                         sfor<ND>([&](auto jd_) {
                             constexpr int jd = decltype(jd_)::value;
-                            acc_add(AKx, tri_index(orow + r, oD + jd), Kd[r][jd], fx);
+                            acc_add<FX>(AKx, tri_index(orow + r, oD + jd), Kd[r][jd]);
                         });
                     }
                 }
@@ -498,9 +502,7 @@ __device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles)
     const int oA = a.sh_ao[sa], oB = a.sh_ao[sb];
     const int g0 = sa - a.sh_local[sa], ns = a.sh_ns[sa];
     // Never reached in the fixed-point mode (the engine keeps no LDS rows then), so the rows go
-    // out as plain FP64 atomics.  (A two-branch acc_add here was miscompiled on gfx950: builds
-    // with it gave wrong, run-dependent exchange rows; tests/test_gpu_async_det.py compares the
-    // two accumulation modes on every class.)
+    // out as plain FP64 atomics.
     const long long copy = (long long)(xcd_id() % a.n_copies) * a.acc_stride + a.sh_tbase[sa];
     unsigned int* mask = (unsigned int*)(tile + n_tiles * (NA + NB) * tn);
     for (int sh = threadIdx.x; sh < ns; sh += blockDim.x) {
@@ -525,7 +527,7 @@ __device__ void flush_rows(const EriArgs& a, double* tile, int pid, int n_tiles)
 }
 
 // The work of one chunk kernel on virtual block vb of nvb (several chunks share a launch).
-template <class R, int C0, bool STORED>
+template <class R, int C0, bool STORED, bool FX>
 __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* tab, double* tile_mem,
                                           long long& s_chunk) {
     const long long lo = a.range[0], hi = a.range[1];
@@ -541,7 +543,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
         for (long long base = first; base < n; base += stride) {
             const long long w = base + threadIdx.x;
             const bool valid = w < n;
-            quartet<R, C0, STORED>(a, lo + (valid ? w : 0), valid, tab, nullptr, -1);
+            quartet<R, C0, STORED, FX>(a, lo + (valid ? w : 0), valid, tab, nullptr, -1);
         }
         return;
     }
@@ -567,7 +569,10 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
         const long long c0 = s_chunk;
         if (c0 >= n) break;
         const long long c1 = c0 + L < n ? c0 + L : n;
-        for (long long base = c0; base < c1; base += blockDim.x) {
+        const long long nit = (c1 - c0 + blockDim.x - 1) / blockDim.x;
+        const long long rot = a.stagger ? (long long)((blockIdx.x * 2654435761u) >> 8) % nit : 0;
+        for (long long it = 0; it < nit; ++it) {
+            const long long base = c0 + (it + rot < nit ? it + rot : it + rot - nit) * blockDim.x;
             if (tile) {
                 // The rows follow the bra pair of the iteration's first entry once it holds
                 // at least a quarter of the iteration.
@@ -583,7 +588,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
             }
             const long long w = base + threadIdx.x;
             const bool valid = w < c1;
-            quartet<R, C0, STORED>(a, lo + (valid ? w : c0), valid, tab, tile, tile_pid);
+            quartet<R, C0, STORED, FX>(a, lo + (valid ? w : c0), valid, tab, tile, tile_pid);
         }
     }
     if (tile && tile_pid >= 0) {
@@ -596,7 +601,7 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
 // virtual block b / NC, so the chunks are interleaved over the machine and run side by side.
 // The class index is a template argument only so that profiles name each class's kernels.
 // STORED: the FP64 stored-integral digest (no Boys table in LDS).
-template <int KC, class R, int CH0, int NC, bool STORED>
+template <int KC, class R, int CH0, int NC, bool STORED, bool FX>
 __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     using S = BoysSlice<LTOT, R>;
     __shared__ __align__(16) R tab[STORED ? 2 : BoysGrid<R>::points * S::nord];
@@ -605,7 +610,7 @@ __global__ __launch_bounds__(256) void k_eri(EriArgs a) {
     const int ch = (int)(blockIdx.x % NC), vb = (int)(blockIdx.x / NC), nvb = (int)(gridDim.x / NC);
     sfor<NC>([&](auto c_) {
         constexpr int c = decltype(c_)::value;
-        if (ch == c) run_chunk<R, (CH0 + c) * Chunks<R>::cb, STORED>(a, vb, nvb, tab, tile_mem, s_chunk);
+        if (ch == c) run_chunk<R, (CH0 + c) * Chunks<R>::cb, STORED, FX>(a, vb, nvb, tab, tile_mem, s_chunk);
     });
 }
 
@@ -633,11 +638,21 @@ void launch_part(int fp32, const EriArgs& a, int grid, const hipStream_t* cs) {
         const int blocks = (grid > NC ? grid / NC : 1) * NC;
         if constexpr (sizeof(R) == 8) {
             if (a.stored) {
-                hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, true>), dim3(blocks), dim3(256), lds, cs[CH_BEGIN], a);
+                if (a.fixed)
+                    hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, true, true>), dim3(blocks), dim3(256), lds,
+                                       cs[CH_BEGIN], a);
+                else
+                    hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, true, false>), dim3(blocks), dim3(256), lds,
+                                       cs[CH_BEGIN], a);
                 return;
             }
         }
-        hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, false>), dim3(blocks), dim3(256), lds, cs[CH_BEGIN], a);
+        if (a.fixed)
+            hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, false, true>), dim3(blocks), dim3(256), lds,
+                               cs[CH_BEGIN], a);
+        else
+            hipLaunchKernelGGL((k_eri<K_COMBO, R, CH_BEGIN, NC, false, false>), dim3(blocks), dim3(256), lds,
+                               cs[CH_BEGIN], a);
     }
 }
 

@@ -490,7 +490,25 @@ __global__ void k_grad_reduce(const double* __restrict__ part, long long stride,
 
 }  // namespace
 
+// Launchers of the compile-time-unrolled classes (k_grad_eri.hip, one translation unit per
+// class), registered at load time.
+GradClassFn& grad_class_fn(int combo) {
+    static GradClassFn table[N_COMBOS] = {};
+    return table[combo];
+}
+
+bool register_grad_class(int combo, GradClassFn fn) {
+    if (combo < 0 || combo >= N_COMBOS) return false;
+    grad_class_fn(combo) = fn;
+    return true;
+}
+
 void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, hipStream_t s) {
+    const int k = combo_of(pair_class(la, lb), pair_class(lc, ld));
+    if (GradClassFn fn = grad_class_fn(k)) {
+        fn(fp32, g, grid, s);
+        return;
+    }
     if (fp32)
         hipLaunchKernelGGL(k_grad_eri<float>, dim3(grid), dim3(64), 0, s, g, la, lb, lc, ld);
     else

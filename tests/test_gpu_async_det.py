@@ -61,7 +61,9 @@ def test_atomic_and_fixed_point_builds_agree_on_every_class(engine_lib, name):
     """The ordinary build (FP64 atomics, LDS exchange rows with flushes, per-XCD copies) and the
     deterministic one (fixed point, no LDS rows) take different digest paths through every class
     kernel; on molecules with thousands of bra-pair switches they must agree to rounding, FP64 and
-    mixed."""
+    mixed.  The two accumulation modes are separately compiled kernels, so in mixed mode their
+    FP32 integrals may differ in the last FP32 bit (about 1e-7 relative of quartets bounded by
+    the cut, 1e-3 here): the mixed comparison allows 1e-9 absolute."""
     from hfx import Basis, geometry
     if name.startswith("C8H18"):
         Z, x = geometry.alkane_chain(n_carbon=8, seed=1003)
@@ -72,12 +74,12 @@ def test_atomic_and_fixed_point_builds_agree_on_every_class(engine_lib, name):
     g = lib.Engine(engine_lib, b)
     D = _density(b.n_aos, b.n_electrons() // 2, 11, scale=0.05)
     print("atomic vs fixed", name, b.n_aos, flush=True)
-    for kw in (dict(), dict(precision=lib.HFX_MIXED, fp32_cut=1e-3)):
+    for kw, tol in ((dict(), 1e-12), (dict(precision=lib.HFX_MIXED, fp32_cut=1e-3), 1e-9)):
         Jd, Kd, _, _ = g.jk(2 * D, D, flags=DET, **kw)
         for flags in (0, lib.HFX_JK_USE_GRAPH, lib.HFX_JK_CLASS_TIMING):
             J, K, _, _ = g.jk(2 * D, D, flags=flags, **kw)
-            assert np.abs(J - Jd).max() <= 1e-12 * max(1.0, np.abs(Jd).max()), (kw, flags)
-            assert np.abs(K - Kd).max() <= 1e-12 * max(1.0, np.abs(Kd).max()), (kw, flags)
+            assert np.abs(J - Jd).max() <= tol * max(1.0, np.abs(Jd).max()), (kw, flags)
+            assert np.abs(K - Kd).max() <= tol * max(1.0, np.abs(Kd).max()), (kw, flags)
 
 
 def test_deterministic_stored_builds(engine_lib):
