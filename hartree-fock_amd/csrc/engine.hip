@@ -2038,17 +2038,26 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
     g.part = e->d_gpart;
     g.part_stride = stride;
     g.n3 = n3;
+    // per-class timing: an event after every class (created for this call only)
+    const bool timing = (a->flags & HFX_JK_CLASS_TIMING) != 0 && st != nullptr;
+    std::vector<hipEvent_t> cev;
+    if (timing) {
+        cev.resize(N_COMBOS);
+        for (auto& ev : cev) HIPCHK(hipEventCreate(&ev));
+    }
     HIPCHK(hipEventRecord(e->ev0, s));
     for (int k = 0; k < N_COMBOS; ++k) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
-        if (e->cls_ids[p1].empty() || e->cls_ids[p2].empty()) continue;
-        const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
-        g.range = e->d_ranges + 2 * k;
-        launch_grad_eri(0, g, la, lb, lc, ld, grid, s);
-        if (mixed) {
-            g.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
-            launch_grad_eri(1, g, la, lb, lc, ld, grid, s);
+        if (!(e->cls_ids[p1].empty() || e->cls_ids[p2].empty())) {
+            const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
+            g.range = e->d_ranges + 2 * k;
+            launch_grad_eri(0, g, la, lb, lc, ld, grid, s);
+            if (mixed) {
+                g.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
+                launch_grad_eri(1, g, la, lb, lc, ld, grid, s);
+            }
         }
+        if (timing) HIPCHK(hipEventRecord(cev[k], s));
     }
     HIPCHK(hipEventRecord(e->ev1, s));
     launch_grad_one(g, grid, s);
@@ -2088,6 +2097,13 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         st->device_ms = ms;
+        if (timing) {
+            for (int k = 0; k < N_COMBOS; ++k) {
+                HIPCHK(hipEventElapsedTime(&ms, k ? cev[k - 1] : e->ev0, cev[k]));
+                st->class_ms[k] = ms;
+            }
+            for (auto& ev : cev) hipEventDestroy(ev);
+        }
     }
     return HFX_OK;
 }

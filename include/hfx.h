@@ -231,13 +231,16 @@ typedef struct hfx_grad_args {
     double density_thresh;     /* and those failing the hfx_jk density test on (P, P/2)      */
     int32_t precision;         /* HFX_FP64, or HFX_MIXED with fp32_cut as in hfx_jk          */
     double fp32_cut;
-    int32_t flags;             /* HFX_JK_DEVICE_PTRS: P and W are device pointers            */
+    int32_t flags;             /* HFX_JK_DEVICE_PTRS: P and W are device pointers;
+                                  HFX_JK_CLASS_TIMING: per-class times in the stats        */
 } hfx_grad_args;
 
 typedef struct hfx_grad_stats {
     int64_t n_quartets;        /* shell quartets differentiated                            */
     int64_t n_fp32;
     double device_ms;          /* device time of the two-electron part                     */
+    double class_ms[21];       /* with HFX_JK_CLASS_TIMING in the flags: device time of each
+                                  quartet class (both precisions), in hfx_jk_stats' order   */
 } hfx_grad_stats;
 
 /* grad[3*n_atoms] (x,y,z per atom).  parts, when not NULL, receives 4 x 3*n_atoms:
