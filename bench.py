@@ -19,7 +19,8 @@ build (one hfx_jk call, dD = D_conv, D_prev = 0) from device-resident inputs.  T
      graph-replayed FP64 builds digesting them (HBM-bound), and the conventional SCF
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Profiling aid: --save-density PATH stores the converged FP64 density (and stops); a later
+Profiling aids: --profile-gradient runs only the (H2O)8 gradient (W + K times, FP64 and mixed);
+--save-density PATH stores the converged FP64 density (and stops); a later
   python bench.py --profile-builds --density PATH --cut C --steps K --warmup W
 runs only the W + K mixed builds with per-class HIP-event timing (no SCF), so a profiler's
 per-kernel averages cover exactly the launches the roofline is computed from; with --stored
@@ -280,11 +281,35 @@ def main():
     ap.add_argument("--cut", type=float, default=None)
     ap.add_argument("--profile-builds", action="store_true")
     ap.add_argument("--stored", action="store_true")
+    ap.add_argument("--profile-gradient", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path (the shell-quartet work "
               "list would be sharded with one all-reduce of J/K per build); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.profile_gradient:
+        # only the (H2O)8 / 6-31G** gradient, FP64 then mixed, W + K times each with per-class
+        # device times (for a profiler's per-kernel averages)
+        L = lib.load()
+        Z, xyz = geometry.water_cluster_grid(shape=(2, 2, 2), spacing=5.6, jitter=0.4, seed=1002)
+        b = Basis("6-31g**", Z, xyz)
+        r = lib.scf_run(L, b, options=L.default_options(energy_tol=1e-9, density_tol=1e-7))
+        P = 2 * r["D_alpha"]
+        W = lib.energy_weighted_density(r["C_alpha"], r["eps_alpha"], sum(b.Z) // 2)
+        e = lib.Engine(L, b)
+        out = {}
+        for name, prec, cut in (("fp64", lib.HFX_FP64, 0.0), ("mixed", lib.HFX_MIXED, 1e-5)):
+            ms = []
+            for it in range(args.warmup + args.steps):
+                _, _, st = lib.rhf_gradient(e, b, P, W, precision=prec, fp32_cut=cut, flags=lib.HFX_JK_CLASS_TIMING)
+                if it >= args.warmup:
+                    ms.append(st["class_ms"])
+            cm = np.mean(ms, axis=0)
+            out[name] = {CLASS_NAMES[i]: round(float(cm[i]), 3) for i in range(21) if cm[i] > 0}
+            out[name + "_two_electron_ms"] = round(float(np.sum(cm)), 3)
+        print(json.dumps({"profile_gradient": args.steps + args.warmup, "quartets": st["n_quartets"], **out}),
+              flush=True)
+        return
     if args.profile_builds:
         if not args.density or (args.cut is None and not args.stored):
             print("--profile-builds needs --density and --cut (or --stored)", file=sys.stderr)

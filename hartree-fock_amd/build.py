@@ -4,6 +4,7 @@ Translation units:
   csrc/engine.hip             host engine, screening / one-electron / symmetrization kernels
   csrc/k_grad.hip             RHF nuclear-gradient kernels (generic two-electron, one-electron, reduction)
   csrc/k_grad_eri.hip (many)  one object per light quartet class: the unrolled two-electron gradient
+  csrc/k_jket.hip (many)      one object per quartet class: the ket-major Coulomb pass of stored builds
   csrc/hfx_scf_gpu.hip        the SCF iteration on the device (rocBLAS/rocSOLVER loaded on first use)
   csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
                               are fully unrolled, so they are compiled in parallel slices
@@ -74,6 +75,8 @@ def jobs():
     srcs.append(("k_grad", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "k_grad.hip")]))
     srcs.append(("hfx_scf_gpu", [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "hfx_scf_gpu.hip")]))
     for (k, la, lb, lc, ld, _) in combo_classes():
+        srcs.append(("k_jket_%02d" % k, [HIPCC] + HIPFLAGS + ["-DHFX_COMBO=%d" % k, "-c",
+                                                               os.path.join(CSRC, "k_jket.hip")]))
         if la + lb + lc + ld <= GRAD_LMAX_UNROLLED:
             srcs.append(("k_grad_eri_%02d" % k, [HIPCC] + HIPFLAGS + ["-DHFX_GRAD_COMBO=%d" % k, "-c",
                                                                       os.path.join(CSRC, "k_grad_eri.hip")]))

@@ -36,6 +36,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "digest.hpp"
 #include "engine.hpp"
 #include "md.hpp"
 #include "scf_device.hpp"
@@ -72,10 +73,20 @@ PartTable& parts() {
     static PartTable t;  // zero-initialized before any dynamic initializer runs
     return t;
 }
+JketFn& jket_fn(int combo) {
+    static JketFn t[N_COMBOS];  // zero-initialized likewise
+    return t[combo];
+}
 
 }  // namespace
 
 namespace hfx {
+
+bool register_jket(int combo, JketFn fn) {
+    if (combo < 0 || combo >= N_COMBOS) return false;
+    jket_fn(combo) = fn;
+    return true;
+}
 
 bool register_eri_part(int combo, int fp32, int begin, int end, int nch, EriPartFn fn) {
     PartTable& t = parts();
@@ -808,6 +819,10 @@ struct hfx_engine {
     uint2* d_slist = nullptr;
     long long* d_sranges = nullptr;  // [2 * N_COMBOS]
     double* d_store = nullptr;
+    // the ket-major copy for the Coulomb pass: per class, entries {ket pair, bra pair} sorted by
+    // ket pair (same ranges as d_slist) and their blocks transposed, in the same layout
+    uint2* d_tlist = nullptr;
+    double* d_storeT = nullptr;
     long long store_n[N_COMBOS] = {}, store_base[N_COMBOS] = {};
     long long store_elems = 0;
     double store_tauS = -1.0;
@@ -823,9 +838,13 @@ struct hfx_engine {
         if (d_slist) (void)hipFree(d_slist);
         if (d_sranges) (void)hipFree(d_sranges);
         if (d_store) (void)hipFree(d_store);
+        if (d_tlist) (void)hipFree(d_tlist);
+        if (d_storeT) (void)hipFree(d_storeT);
         d_slist = nullptr;
         d_sranges = nullptr;
         d_store = nullptr;
+        d_tlist = nullptr;
+        d_storeT = nullptr;
         for (int k = 0; k < N_COMBOS; ++k) store_n[k] = store_base[k] = 0;
         store_elems = 0;
         store_tauS = -1.0;
@@ -1578,6 +1597,13 @@ int enqueue_build(hfx_engine* e, const BuildParams& P, hipStream_t s, int* ev_sl
         ea.chunk_ctr = e->d_chunk_ctr + (2 * k) * hfx_engine::CHUNK_SLOTS;
         ea.tile_n = tile_cols(k, 0);
         launch_eri(k, 0, ea, grid, cs[2 * k].data());
+        if (P.stored) {
+            // the ket-side Coulomb terms from the ket-major copy, on the stream of chunk 0
+            EriArgs ta = ea;
+            ta.list = e->d_tlist;
+            ta.stored = e->d_storeT + e->store_base[k];
+            jket_fn(k)(ta, grid, cs[2 * k][0]);
+        }
         if (timing) {
             HIPCHK(hipEventRecord(e->cev[n_ev], s));
             ev_slot[2 * k] = n_ev++;
@@ -1855,6 +1881,75 @@ int jk_async_impl(hfx_engine* e, const hfx_jk_args* args, double* J, double* Ka,
 // test passing (zero density, tau_D = 0), copied into a list of their own, and their normalized
 // blocks computed once by the store mode of the FP64 class kernels, in the layout the stored
 // digest reads (element-major groups of 64 entries).  Replaces any earlier store.
+// Device memory owned by one scope (temporary buffers of a set-up step).
+struct DeviceBuffer {
+    void* p = nullptr;
+    explicit DeviceBuffer(size_t bytes) {
+        if (hipMalloc(&p, bytes) != hipSuccess) p = nullptr;
+    }
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+};
+
+// Ket-major copy of the stored integrals: per class, the entries sorted by ket pair (stable, so
+// the bras of one ket stay in list order), the list {ket pair, bra pair} and the blocks
+// transposed (element cd*NAB + ab of entry r' at ((r'/64) NAB NCD + cd NAB + ab) 64 + r' mod 64).
+__global__ void k_ket_keys(const uint2* __restrict__ list, long long n, unsigned int* keys, unsigned int* idx) {
+    for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
+        keys[r] = list[r].y;
+        idx[r] = (unsigned int)r;
+    }
+}
+__global__ void k_ket_transpose(const uint2* __restrict__ list, const unsigned int* __restrict__ perm, long long n,
+                                int nab, int ncd, const double* __restrict__ src, uint2* tlist, double* dst) {
+    const long long blk = (long long)nab * ncd;
+    const long long total = n * blk;
+    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < total;
+         x += (long long)gridDim.x * blockDim.x) {
+        // consecutive threads: consecutive destination entries of one transposed element
+        const long long et = x / n, rt = x - et * n;
+        const long long r = perm[rt];
+        const long long cd = et / nab, ab = et - cd * nab;
+        const long long es = ab * ncd + cd;
+        dst[((rt >> 6) * blk + et) * 64 + (rt & 63)] = src[((r >> 6) * blk + es) * 64 + (r & 63)];
+        if (et == 0) tlist[rt] = make_uint2(list[r].y, list[r].x);
+    }
+}
+
+int store_ket_major(hfx_engine* e, const long long* sr, hipStream_t s) {
+    long long nmax = 0;
+    for (int k = 0; k < N_COMBOS; ++k) nmax = std::max(nmax, e->store_n[k]);
+    if (nmax == 0) return HFX_OK;
+    if (nmax >= (1LL << 32)) return fail(HFX_ERR_UNSUPPORTED, "hfx_store_eris: more than 2^32 quartets in one class");
+    DeviceBuffer keys(4 * nmax), keys2(4 * nmax), idx(4 * nmax), idx2(4 * nmax);
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (unsigned int*)keys.p, (unsigned int*)keys2.p,
+                                              (unsigned int*)idx.p, (unsigned int*)idx2.p, (int)nmax, 0, 32, s));
+    DeviceBuffer tmp(std::max<size_t>(tmp_bytes, 16));
+    if (!keys.p || !keys2.p || !idx.p || !idx2.p || !tmp.p) return fail(HFX_ERR_NOMEM, "hfx_store_eris: hipMalloc failed");
+    const int g = e->n_cu * 8;
+    for (int k = 0; k < N_COMBOS; ++k) {
+        const long long n = e->store_n[k];
+        if (n == 0) continue;
+        const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+        const int nab = ncart_h(pc_la(p1)) * ncart_h(pc_lb(p1)), ncd = ncart_h(pc_la(p2)) * ncart_h(pc_lb(p2));
+        const uint2* list = e->d_slist + sr[2 * k];
+        hipLaunchKernelGGL(k_ket_keys, dim3(g), dim3(256), 0, s, list, n, (unsigned int*)keys.p, (unsigned int*)idx.p);
+        size_t tb = tmp_bytes;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, (unsigned int*)keys.p, (unsigned int*)keys2.p,
+                                                  (unsigned int*)idx.p, (unsigned int*)idx2.p, (int)n, 0, 32, s));
+        hipLaunchKernelGGL(k_ket_transpose, dim3(g), dim3(256), 0, s, list, (const unsigned int*)idx2.p, n, nab, ncd,
+                           (const double*)(e->d_store + e->store_base[k]), e->d_tlist + sr[2 * k],
+                           e->d_storeT + e->store_base[k]);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));  // the temporary buffers are freed on return
+    return HFX_OK;
+}
+
 int store_impl(hfx_engine* e, const double* Q, double tauS, int64_t* n_out, int64_t* bytes_out) {
     if (!e) return fail(HFX_ERR_INVALID, "hfx_store_eris: null engine");
     if (!(tauS >= 0.0) || !std::isfinite(tauS)) return fail(HFX_ERR_INVALID, "hfx_store_eris: bad schwarz_thresh");
@@ -1892,7 +1987,8 @@ int store_impl(hfx_engine* e, const double* Q, double tauS, int64_t* n_out, int6
     }
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    const double need = (double)elems * sizeof(double) + (double)n_tot * sizeof(uint2);
+    // the blocks twice (bra-major, and the ket-major copy of the Coulomb pass) and the lists
+    const double need = 2.0 * (double)elems * sizeof(double) + 2.0 * (double)n_tot * sizeof(uint2);
     if (need + (1ull << 30) > (double)free_b) {
         e->release_store();
         return fail(HFX_ERR_NOMEM, "hfx_store_eris: the stored integrals need " + std::to_string(need / 1e9) +
@@ -1900,7 +1996,9 @@ int store_impl(hfx_engine* e, const double* Q, double tauS, int64_t* n_out, int6
     }
     if (hipMalloc(&e->d_slist, std::max<long long>(1, n_tot) * sizeof(uint2)) != hipSuccess ||
         hipMalloc(&e->d_sranges, sizeof(sr)) != hipSuccess ||
-        hipMalloc(&e->d_store, std::max<long long>(1, elems) * sizeof(double)) != hipSuccess) {
+        hipMalloc(&e->d_store, std::max<long long>(1, elems) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&e->d_tlist, std::max<long long>(1, n_tot) * sizeof(uint2)) != hipSuccess ||
+        hipMalloc(&e->d_storeT, std::max<long long>(1, elems) * sizeof(double)) != hipSuccess) {
         e->release_store();
         return fail(HFX_ERR_NOMEM, "hfx_store_eris: hipMalloc failed");
     }
@@ -1916,7 +2014,7 @@ int store_impl(hfx_engine* e, const double* Q, double tauS, int64_t* n_out, int6
     a.tauS = tauS;
     for (int k = 0; k < N_COMBOS; ++k) {
         if (e->store_n[k] == 0) continue;
-        if (eri_chunks(k, 0) < 0) {
+        if (eri_chunks(k, 0) < 0 || !jket_fn(k)) {
             e->release_store();
             return fail(HFX_ERR_UNSUPPORTED, "ERI kernels missing for class " + std::to_string(k));
         }
@@ -1926,6 +2024,13 @@ int store_impl(hfx_engine* e, const double* Q, double tauS, int64_t* n_out, int6
         launch_eri(k, 0, a, eri_grid(e), cs.data());
     }
     HIPCHK(hipGetLastError());
+    {
+        const int rc = store_ket_major(e, sr, s);
+        if (rc) {
+            e->release_store();
+            return rc;
+        }
+    }
     HIPCHK(hipStreamSynchronize(s));
     e->store_tauS = tauS;
     if (n_out) *n_out = n_tot;
@@ -2102,7 +2207,7 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
                 HIPCHK(hipEventElapsedTime(&ms, k ? cev[k - 1] : e->ev0, cev[k]));
                 st->class_ms[k] = ms;
             }
-            for (auto& ev : cev) hipEventDestroy(ev);
+            for (auto& ev : cev) (void)hipEventDestroy(ev);
         }
     }
     return HFX_OK;

@@ -154,6 +154,10 @@ void launch_grad_reduce(const GradArgs& g, int n_rows, double* out, hipStream_t 
 // one launch (on the stream of its first chunk, chunk_streams[begin]).
 // A class has nch chunks in each precision (fewer in FP32 for the heavy classes).
 using EriPartFn = void (*)(int fp32, const EriArgs& a, int grid, const hipStream_t* chunk_streams);
+// The ket-major Coulomb pass of a class's stored integrals (k_jket.hip, one translation unit per
+// class), registered at load time.
+using JketFn = void (*)(const EriArgs& a, int grid, hipStream_t s);
+bool register_jket(int combo, JketFn fn);
 bool register_eri_part(int combo, int fp32, int begin, int end, int nch, EriPartFn fn);
 // Number of chunks of a class in a precision, or -1 when its parts do not cover [0, nch).
 int eri_chunks(int combo, int fp32);

@@ -37,6 +37,9 @@ double trace_prod(int n, const double* A, const double* B) {  // tr(A B)
     return s;
 }
 
+// Orbital energy given to the directions outside an orbital subspace.
+constexpr double REMOVED_EPS = 1.0e6;
+
 double frob(const Mat& A) {
     double s = 0.0;
     for (double x : A) s += x * x;
@@ -52,6 +55,20 @@ Mat inv_sqrt(int n, const Mat& S) {
         for (int r = 0; r < n; ++r) T[r + (size_t)c * n] = V[r + (size_t)c * n] * f;
     }
     hfx_la::mm(n, T.data(), V.data(), X.data(), false, true);
+    return X;
+}
+
+// Orthogonalizer of the orbital space spanned by the M columns of Tm (n x M, column-major):
+// X = Tm (Tm^T S Tm)^-1/2 in the first M columns, zero columns after them (n x n).  With
+// Tm = 1 this is inverseSqrtMatrix (Utils.cpp:117-138); with the Cartesian-to-spherical
+// transform of the shells, the SCF runs in the spherical-harmonic orbital space while every
+// matrix stays in the engine's Cartesian AO basis.
+Mat subspace_inv_sqrt(int n, int M, const double* Tm, const Mat& S) {
+    Mat ST((size_t)n * M), Ss((size_t)M * M), X((size_t)n * n, 0.0);
+    hfx_la::gemm(false, false, n, M, n, 1.0, S.data(), n, Tm, n, 0.0, ST.data(), n);
+    hfx_la::gemm(true, false, M, M, n, 1.0, Tm, n, ST.data(), n, 0.0, Ss.data(), M);
+    const Mat Xs = inv_sqrt(M, Ss);
+    hfx_la::gemm(false, false, n, M, M, 1.0, Tm, n, Xs.data(), M, 0.0, X.data(), n);
     return X;
 }
 
@@ -106,6 +123,7 @@ struct Scf {
     hfx_engine* e = nullptr;
     hfx_scf_options o{};
     int n = 0, nS = 0, na = 0, nb = 0;
+    int m = 0;  // orbitals: n, or the M of an orbital subspace (opt->orbital_space)
     bool uhf = false;
     double e_nuc = 0, e_elec = 0, dE = 0, dD = 0, diis_err = 0, damp_c = 0;
     bool ls_a = false, ls_b = false;
@@ -138,7 +156,26 @@ struct Scf {
         }
         Mat V = mat();
         w.assign(n, 0.0);
-        hfx_la::eigh(n, Fp.data(), w.data(), V.data());
+        if (m < n) {
+            // an orbital subspace: X has zero columns after the first m, so the eigenproblem is
+            // the leading m x m block; the other directions keep zero coefficient columns and a
+            // marker orbital energy above the spectrum
+            Mat Fs((size_t)m * m), Vs((size_t)m * m);
+            std::vector<double> ws(m);
+            for (int c = 0; c < m; ++c)
+                for (int r = 0; r < m; ++r) Fs[r + (size_t)c * m] = Fp[r + (size_t)c * n];
+            hfx_la::eigh(m, Fs.data(), ws.data(), Vs.data());
+            for (int c = 0; c < m; ++c) {
+                w[c] = ws[c];
+                for (int r = 0; r < m; ++r) V[r + (size_t)c * n] = Vs[r + (size_t)c * m];
+            }
+            for (int c = m; c < n; ++c) {
+                w[c] = REMOVED_EPS;
+                V[c + (size_t)c * n] = 1.0;
+            }
+        } else {
+            hfx_la::eigh(n, Fp.data(), w.data(), V.data());
+        }
         C = mat();
         hfx_la::mm(n, X.data(), V.data(), C.data());
         t_diag += secs(t0, clk::now());
@@ -232,6 +269,17 @@ struct Scf {
             hfx_la::mm(n, S.data(), Db.data(), t1.data());
             hfx_la::mm(n, t1.data(), Fb.data(), eb.data());
             for (size_t x = 0; x < eb.size(); ++x) eb[x] = t2[x] - eb[x] + ea[x];
+            if (m < n) {
+                // in an orbital subspace the AO commutator need not vanish at convergence; its
+                // S-projection P^T (FDS - SDF) P onto the subspace (P = X X^T S) does.  For the
+                // full space P = 1 and this is the AO commutator itself, so DIIS keeps the
+                // reference's metric.
+                hfx_la::mm(n, eb.data(), X.data(), t1.data());
+                hfx_la::mm(n, X.data(), t1.data(), t2.data(), true, false);  // X^T e X
+                hfx_la::mm(n, S.data(), X.data(), ea.data());                // S X
+                hfx_la::mm(n, ea.data(), t2.data(), t1.data());
+                hfx_la::mm(n, t1.data(), ea.data(), eb.data(), false, true);
+            }
             diis->update(Fa, Fb, eb);
             diis->extrapolate(Fa, Fb);
         }
@@ -299,7 +347,7 @@ struct Scf {
         Da = density(Ca, na);
         Db_prev = mat();
         if (uhf) {
-            if (o.guess_mix > 0 && na >= 1 && na < n) {
+            if (o.guess_mix > 0 && na >= 1 && na < m) {
                 const double k = o.guess_mix / 10.0;
                 const double f = 1.0 / std::sqrt(1 + k * k);
                 for (int r = 0; r < n; ++r) {
@@ -352,9 +400,9 @@ struct Scf {
 
     void remove_level_shift() {  // SCF.cpp:93-107
         if (ls_a)
-            for (int i = na; i < n; ++i) eps_a[i] -= o.level_shift;
+            for (int i = na; i < m; ++i) eps_a[i] -= o.level_shift;
         if (ls_b)
-            for (int i = nb; i < n; ++i) eps_b[i] -= o.level_shift;
+            for (int i = nb; i < m; ++i) eps_b[i] -= o.level_shift;
     }
 
     void write_result(hfx_scf_result* res) const {
@@ -369,6 +417,7 @@ struct Scf {
         res->n_jk_builds = n_builds;
         res->n_quartets_last = n_quart_last;
         res->time_linalg_load_s = 0.0;
+        res->n_mo = m;
         auto put = [&](double* dst, const Mat& src) {
             if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
         };
@@ -392,7 +441,12 @@ struct Scf {
         if (mult < 1 || nel < 0 || ((nel + mult - 1) % 2) != 0 || nel + 1 < mult) return HFX_ERR_INVALID;
         na = (int)((nel + mult - 1) / 2);
         nb = (int)((nel - mult + 1) / 2);
-        if (na > n) return HFX_ERR_INVALID;
+        m = n;
+        if (o.orbital_space) {
+            if (o.n_orbital_space <= 0 || o.n_orbital_space > n) return HFX_ERR_INVALID;
+            m = o.n_orbital_space;
+        }
+        if (na > m) return HFX_ERR_INVALID;
         uhf = o.unrestricted < 0 ? (mult != 1) : (o.unrestricted != 0);
         precision = o.precision;
         double en = 0.0;
@@ -491,6 +545,9 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
     }
     s.h = s.mat();
     for (size_t x = 0; x < s.h.size(); ++x) s.h[x] = T[x] + V[x];
+    if (s.o.device_linalg && s.o.orbital_space)
+        return hfx_internal::set_error(HFX_ERR_UNSUPPORTED, "hfx_scf_run: orbital_space needs the host linear algebra "
+                                                           "(device_linalg = 0)");
     if (s.o.device_linalg) {  // the cycles with the iteration's linear algebra on the device
         const hfx_internal::ScfSetup su{s.n, s.nS, s.na, s.nb, s.uhf, s.precision, s.e_nuc, s.S.data(), s.h.data(),
                                         s.Q.data()};
@@ -500,7 +557,7 @@ int hfx_scf_run_engine(hfx_engine* e, int32_t n_atoms, const double* Z, const do
         res->time_total_s = secs(t0, clk::now());
         return HFX_OK;
     }
-    s.X = inv_sqrt(s.n, s.S);
+    s.X = s.o.orbital_space ? subspace_inv_sqrt(s.n, s.m, s.o.orbital_space, s.S) : inv_sqrt(s.n, s.S);
     s.initial_guess();
     const auto t_init = clk::now();
 
@@ -567,6 +624,9 @@ int hfx_scf_run_batched_engine(hfx_engine* e, int32_t n_mol, const int32_t* n_ao
     int rc = hfx_sizes(e, &N, &nS, &nm, &elems);
     if (rc) return rc;
     if (nm != n_mol) return HFX_ERR_INVALID;
+    if (opt && opt->orbital_space)
+        return hfx_internal::set_error(HFX_ERR_UNSUPPORTED, "hfx_scf_run_batched: orbital_space is per molecule; "
+                                                           "run such molecules one at a time");
     hfx_scf_options o;
     if (opt)
         o = *opt;

@@ -753,6 +753,7 @@ int device_cycles(hfx_engine* e, const hfx_scf_options& o, const ScfSetup& su, h
     res->converged = conv ? 1 : 0;
     res->time_init_s = secs(t0, t_init);
     res->time_linalg_load_s = load_s;
+    res->n_mo = su.n;
     return HFX_OK;
 }
 

@@ -29,6 +29,7 @@
 // triangles (both orientations of (i,j) land in one element, i.e. J = AJ + AJ^T is formed
 // on the fly; the diagonal is doubled when the triangles are expanded).  This equals the reference's canonical-AO digestion
 // (src/SCF.cpp:633-711) summed over the 8 permutations.
+#include "digest.hpp"
 #include "engine.hpp"
 #include "md.hpp"
 
@@ -43,6 +44,7 @@ namespace hfx {
 namespace {
 
 using namespace md;
+using namespace digest;
 
 constexpr int K_COMBO = HFX_COMBO;
 constexpr int PC1 = combo_pc1(K_COMBO), PC2 = combo_pc2(K_COMBO);
@@ -83,40 +85,6 @@ template <>
 __device__ __forceinline__ const double* boys_of<double>(const EriArgs& a) { return a.boys64; }
 template <>
 __device__ __forceinline__ const float* boys_of<float>(const EriArgs& a) { return a.boys32; }
-
-// Accumulation into the copy private to this XCD (no other XCD touches these lines): FP64
-// atomics, or in the deterministic mode the fixed-point pairs below.
-// Deterministic accumulation: v as a two-word fixed-point number, hi = round(v 2^32) and
-// lo = round((v 2^32 - hi) 2^40), each added with an integer atomic.  Integer sums do not
-// depend on the order of the additions, so the result is bitwise reproducible; the rounding
-// error of one term is at most 2^-73 and the range is |sum| < 2^31.
-__device__ __forceinline__ void fixed_add(unsigned long long* q, double v) {
-    const double x = ldexp(v, 32);
-    const double r = rint(x);
-    const long long hi = (long long)r;
-    const long long lo = (long long)rint(ldexp(x - r, 40));
-    if (hi) __hip_atomic_fetch_add(q, (unsigned long long)hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lo) __hip_atomic_fetch_add(q + 1, (unsigned long long)lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Element t of an accumulator block: a double (FX false) or a fixed-point pair at [2t, 2t+1]
-// (FX true).  The accumulation mode is a kernel template parameter, not a run-time flag: the
-// fixed-point path's temporaries would otherwise count in every kernel's register allocation
-// at the digest's peak (dsps: one wavefront per SIMD instead of two).
-template <bool FX>
-__device__ __forceinline__ void acc_add(double* base, long long t, double v) {
-    if (v == 0.0) return;
-    if constexpr (!FX)
-        __hip_atomic_fetch_add(base + t, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else
-        fixed_add(reinterpret_cast<unsigned long long*>(base) + 2 * t, v);
-}
-// Accumulation into the block's LDS exchange rows.
-__device__ __forceinline__ void lds_add(double* p, double v) {
-    if (v != 0.0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Index of the XCD running this workgroup (hardware register XCC_ID, bits 3:0).
-__device__ __forceinline__ int xcd_id() { return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15u); }
 
 // Is bra component row ia (of shell a) / ib (of shell b) touched by chunk C0?
 __host__ __device__ constexpr bool chunk_has_a(int c0, int cb, int ia) {
@@ -374,7 +342,7 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                         const long long i = oA + ab / NB, j = oB + ab % NB;
                         const double v = val(c_, cd_);
                         Jab[c] += v * dkl;
-                        Jcd[cd] += v * Dt[tri_index(i, j)];
+                        if constexpr (!STORED) Jcd[cd] += v * Dt[tri_index(i, j)];
                     }
                 });
             });
@@ -406,10 +374,13 @@ __device__ __forceinline__ void quartet(const EriArgs& a, long long qi, bool val
                     if (head && valid) acc_add<FX>(AJ, tri_index(oA + ab / NB, oB + ab % NB), 2.0 * x);
                 }
             });
-            sfor<NCD>([&](auto cd_) {
-                constexpr int cd = decltype(cd_)::value;
-                acc_add<FX>(AJ, tri_index(oC + cd / ND, oD + cd % ND), 2.0 * Jcd[cd]);
-            });
+            // The stored digest leaves the ket-side Coulomb terms to the ket-major pass (k_jket.hip).
+            if constexpr (!STORED) {
+                sfor<NCD>([&](auto cd_) {
+                    constexpr int cd = decltype(cd_)::value;
+                    acc_add<FX>(AJ, tri_index(oC + cd / ND, oD + cd % ND), 2.0 * Jcd[cd]);
+                });
+            }
         }
         // Exchange with one spin density Dx into the accumulator AKx (rows of a and b into the
         // LDS rows T instead when the block holds this bra pair).

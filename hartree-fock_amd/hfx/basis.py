@@ -193,6 +193,41 @@ class Basis:
         )
         return self._flat
 
+    def spherical_transform(self):
+        """Cartesian-to-spherical transform T (n_aos x n_sph, column-major order of the shells):
+        the columns of a shell are its real solid harmonics in terms of the shell's Cartesian
+        functions as the engine normalizes them (every component normalized by itself,
+        ``Basis.cpp:47-77``).  s and p: the Cartesian functions themselves (p in x, y, z
+        order).  d, m = -2..2: xy, yz, (2zz - xx - yy)/2, xz, sqrt(3)/2 (xx - yy); with the
+        components normalized alike, <xx|yy> = <xx|xx>/3, so every column has the norm of the
+        shell's xy component and the five are orthogonal.  The reference has no spherical
+        option; this is the orbital space of ``hfx_scf_options.orbital_space``."""
+        blocks = []
+        h = math.sqrt(3.0) / 2.0
+        d = np.array([[0, 0, -0.5, 0, h],     # xx
+                      [1, 0, 0, 0, 0],        # xy
+                      [0, 0, 0, 1, 0],        # xz
+                      [0, 0, -0.5, 0, -h],    # yy
+                      [0, 1, 0, 0, 0],        # yz
+                      [0, 0, 1, 0, 0]],       # zz
+                     dtype=np.float64)
+        for sh in self.shells:
+            if sh.l <= 1:
+                blocks.append(np.eye(sh.nao))
+            elif sh.l == 2:
+                blocks.append(d)
+            else:
+                raise ValueError("spherical_transform: l <= 2 only")
+        n = self.n_aos
+        m = sum(b.shape[1] for b in blocks)
+        T = np.zeros((n, m))
+        r = c = 0
+        for b in blocks:
+            T[r:r + b.shape[0], c:c + b.shape[1]] = b
+            r += b.shape[0]
+            c += b.shape[1]
+        return T
+
     def nuclear_repulsion(self):
         """``Molecule::nuclearRepulsion`` (reference ``src/Molecule.cpp:75-89``)."""
         e = 0.0

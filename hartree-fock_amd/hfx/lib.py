@@ -87,7 +87,8 @@ class HfxScfOptions(C.Structure):
                 ("max_lshift_iter", C.c_int32), ("stop_lshift_thresh", C.c_double), ("precision", C.c_int32),
                 ("fp32_cut", C.c_double), ("final_fp64", C.c_int32), ("jk_flags", C.c_int32),
                 ("full_builds", C.c_int32), ("direct", C.c_int32),
-                ("switch_tol", C.c_double), ("device_linalg", C.c_int32)]
+                ("switch_tol", C.c_double), ("device_linalg", C.c_int32), ("orbital_space", _f64p),
+                ("n_orbital_space", C.c_int32)]
 
 
 class HfxScfResult(C.Structure):
@@ -100,7 +101,7 @@ class HfxScfResult(C.Structure):
                 ("F_alpha", _f64p), ("D_alpha", _f64p), ("C_alpha", _f64p), ("eps_alpha", _f64p),
                 ("F_beta", _f64p), ("D_beta", _f64p), ("C_beta", _f64p), ("eps_beta", _f64p),
                 ("h_core", _f64p), ("S", _f64p), ("trace", _f64p), ("trace_cap", C.c_int32),
-                ("time_linalg_load_s", C.c_double)]
+                ("time_linalg_load_s", C.c_double), ("n_mo", C.c_int32)]
 
 
 ABI_SYMBOLS = ["hfx_create", "hfx_create_batched", "hfx_destroy", "hfx_sizes", "hfx_overlap", "hfx_kinetic",
@@ -379,13 +380,23 @@ def jk_async(engine, dD_tot, dD_alpha, J, Ka, stream, dD_beta=None, Kb=None, Q=N
 
 
 def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, outputs=True, trace_cap=200,
-            engine=None):
+            engine=None, spherical=False):
     """Run the library's SCF driver (hfx_scf_run, or hfx_scf_run_engine over `engine`); returns a
-    dict with energies and (optionally) matrices."""
+    dict with energies and (optionally) matrices.  spherical=True runs in the orbital space of
+    the spherical-harmonic functions of the shells (Basis.spherical_transform); the matrices
+    stay in the Cartesian AO basis."""
     o = options if options is not None else library.default_options()
     r = HfxScfResult()
     n = basis.n_aos
     keep = {}
+    if spherical:
+        T = np.asfortranarray(basis.spherical_transform())
+        keep["_T"] = T
+        o2 = HfxScfOptions()
+        C.memmove(C.byref(o2), C.byref(o), C.sizeof(o))
+        o2.orbital_space = T.ctypes.data_as(_f64p)
+        o2.n_orbital_space = T.shape[1]
+        o = o2
     if outputs:
         for k in ("F_alpha", "D_alpha", "C_alpha", "F_beta", "D_beta", "C_beta", "h_core", "S"):
             keep[k] = np.zeros(n * n)
@@ -411,7 +422,7 @@ def scf_run(library, basis, charge=0, multiplicity=1, options=None, device=0, ou
     it = r.iterations
     out["trace"] = keep["trace"][:4 * min(it, trace_cap)].reshape(-1, 4)
     for k, v in keep.items():
-        if k == "trace":
+        if k in ("trace", "_T"):
             continue
         out[k] = v.reshape(n, n, order="F") if v.size == n * n else v
     return out

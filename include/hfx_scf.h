@@ -56,6 +56,14 @@ typedef struct hfx_scf_options {
                                /* stream-ordered through hfx_jk_async; the host reads a few   */
                                /* scalars per iteration.  Single-molecule runs; diis_size     */
                                /* 1..15.  0 (default): the host driver's own linear algebra   */
+    const double* orbital_space; /* NULL (default): the orbitals span all N AOs.  Otherwise an */
+                                 /* N x M column-major matrix whose columns span the orbital  */
+                                 /* space, e.g. the Cartesian-to-spherical transform of the    */
+                                 /* shells (hfx.Basis.spherical_transform): X = T (T^T S T)^-1/2, */
+                                 /* every matrix stays in the Cartesian AO basis, the DIIS     */
+                                 /* error is X^T (FDS - SDF) X.  Host linear algebra, single-  */
+                                 /* molecule runs only                                        */
+    int32_t n_orbital_space;     /* M, 1 <= M <= N                                            */
 } hfx_scf_options;
 
 typedef struct hfx_scf_result {
@@ -84,6 +92,9 @@ typedef struct hfx_scf_result {
     double time_linalg_load_s;   /* device_linalg: seconds spent loading rocBLAS/rocSOLVER   */
                                  /* and creating the device's handle (first run in a process */
                                  /* only; inside time_init_s and time_total_s)               */
+    int32_t n_mo;                /* orbitals: N, or M with an orbital_space; C and eps keep  */
+                                 /* N columns, the last N - M with zero coefficients and an  */
+                                 /* orbital energy of 1e6                                     */
 } hfx_scf_result;
 
 void hfx_scf_default_options(hfx_scf_options* opt);
