@@ -142,6 +142,64 @@ def batched_run(L, cut, n_mol=64):
                 converged=sum(a["converged"] for a in rmx), max_dE_final=max(dE), max_dE_at_switch=max(dEm))
 
 
+def eri_model_flop(la, lb, lc, ld):
+    """Model FLOPs of one primitive quartet of the J/K build (the engine's count: Boys values and
+    R^0_tuv to level L, the bra-side Hermite sums for every ket Hermite index, the ket-side sums
+    for every bra component, and the final products), with the cheaper of the two orientations."""
+    def nherm(l):
+        return (l + 1) * (l + 2) * (l + 3) // 6
+
+    def ncart(l):
+        return (l + 1) * (l + 2) // 2
+
+    def hsum(l1, l2):
+        s = 0
+        for i1 in range(l1, -1, -1):
+            for j1 in range(l1 - i1, -1, -1):
+                for i2 in range(l2, -1, -1):
+                    for j2 in range(l2 - i2, -1, -1):
+                        s += (i1 + i2 + 1) * (j1 + j2 + 1) * ((l1 - i1 - j1) + (l2 - i2 - j2) + 1)
+        return s
+
+    def one(a, b, c, d):
+        L = a + b + c + d
+        c4 = (L + 4) * (L + 3) * (L + 2) * (L + 1) / 24.0
+        return (20 + 3 * (L + 1)) + 3 * c4 + 2.0 * nherm(a + b) * hsum(c, d) + \
+            2.0 * ncart(c) * ncart(d) * hsum(a, b) + 2.0 * ncart(a) * ncart(b) * ncart(c) * ncart(d)
+    return min(one(la, lb, lc, ld), one(lc, ld, la, lb))
+
+
+def grad_model_flop(la, lb, lc, ld):
+    """Model FLOPs of one primitive quartet of the two-electron gradient: the derivative of (ab|cd)
+    on centre A is 2a (a+1 b|cd) - a_x (a-1 b|cd), and likewise on B and C (D follows from
+    translational invariance), so the gradient needs the three classes with one shell raised, each
+    counted with the J/K build's model (eri_model_flop; the lowered classes are their subsets), plus
+    the contraction of the nine derivative blocks with Gamma (2 FLOPs per element).  A count of the
+    algorithm, not of the kernel's instructions (the kernels re-form the ket sums per component pair)."""
+    ncomp = ((la + 1) * (la + 2) // 2) * ((lb + 1) * (lb + 2) // 2) * ((lc + 1) * (lc + 2) // 2) * \
+        ((ld + 1) * (ld + 2) // 2)
+    return (eri_model_flop(la + 1, lb, lc, ld) + eri_model_flop(la, lb + 1, lc, ld) +
+            eri_model_flop(la, lb, lc + 1, ld) + 18.0 * ncomp)
+
+
+def gradient_roofline(st):
+    """Roofline of the gradient's dominant class (its kernels, one class): model FLOPs of its
+    primitive quartets (grad_model_flop) over its device time, against the FP64 vector peak."""
+    cms, prims = st["class_ms"], st["class_prims"]
+    k = int(np.argmax(cms))
+    gfl = float(prims[k]) * grad_model_flop(*CLASS_L[k]) * 1e-9
+    achieved = gfl / cms[k] if cms[k] > 0 else 0.0
+    total = sum(float(prims[i]) * grad_model_flop(*CLASS_L[i]) for i in range(21)) * 1e-9
+    kern = "k_grad_eri<double> (generic)" if CLASS_NAMES[k] in ("dddp", "dddd") else "k_grad_eri_t<double, c0>"
+    per = {CLASS_NAMES[i]: round(float(prims[i]) * grad_model_flop(*CLASS_L[i]) * 1e-9 / cms[i], 3)
+           for i in range(21) if cms[i] > 0 and prims[i] > 0}
+    return dict(bound="valu", pipe="VALU f64", kernel="%s %s" % (kern, CLASS_NAMES[k]), achieved=achieved,
+                peak=PEAK_TFLOPS["f64"], unit="TFLOP/s", frac=achieved / PEAK_TFLOPS["f64"], traffic=None,
+                launch_ms=float(cms[k]), launch_model_gflop=gfl, prim_quartets=int(prims[k]),
+                two_electron_model_gflop=total, two_electron_tflops=total / float(np.sum(cms)),
+                class_model_tflops=per)
+
+
 def gradient_run(L):
     Z, xyz = geometry.water_cluster_grid(shape=(2, 2, 2), spacing=5.6, jitter=0.4, seed=1002)
     b = Basis("6-31g**", Z, xyz)
@@ -156,10 +214,13 @@ def gradient_run(L):
     t = time.perf_counter()
     gmx, _, smx = lib.rhf_gradient(e, b, P, W, precision=lib.HFX_MIXED, fp32_cut=1e-5)
     tmx = time.perf_counter() - t
+    _, _, stc = lib.rhf_gradient(e, b, P, W, flags=lib.HFX_JK_CLASS_TIMING)  # per-class device times
     return dict(system="(H2O)8/6-31G** (%d functions)" % b.n_aos, seconds_fp64=t64, seconds_mixed=tmx,
                 two_electron_ms_fp64=s64["device_ms"], two_electron_ms_mixed=smx["device_ms"],
                 quartets=s64["n_quartets"], fp32_quartets=smx["n_fp32"],
-                max_dgrad_mixed=float(np.abs(gmx - g64).max()), max_abs_grad=float(np.abs(g64).max()))
+                max_dgrad_mixed=float(np.abs(gmx - g64).max()), max_abs_grad=float(np.abs(g64).max()),
+                class_ms_fp64={CLASS_NAMES[i]: round(float(stc["class_ms"][i]), 3) for i in range(21)},
+                roofline=gradient_roofline(stc))
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eri_c20h42_mixed_traffic.json")
@@ -391,10 +452,14 @@ def main():
     Jmx, Kmx = bufs[2].download(), bufs[3].download()
     log("builds/s: FP64 eager %.3f graph %.3f | mixed eager %.3f graph %.3f" % (
         args.steps / t64e, args.steps / t64, args.steps / tmxe, args.steps / tmx))
-    eng.set_launch(*launch64)
+    # Per-class times (and the roofline's kernel duration) one class at a time, at the engine's
+    # default launch (2 blocks per CU, runs of 2048), the configuration of --profile-builds and of
+    # the rocprofv3 summaries: a lone kernel at the tuner's 1 block per CU would leave three of the
+    # four blocks per CU that the overlapped build gives it empty.
+    eng.set_launch(2, 2048, 4)
     _, st64, c64, _ = timed_builds(L, eng, bufs, n, args.steps, 1, lib.HFX_FP64, 0.0, Q)
-    eng.set_launch(*launchmx)
     _, stmx, cm64, cm32 = timed_builds(L, eng, bufs, n, args.steps, 1, lib.HFX_MIXED, cut, Q)
+    eng.set_launch(*launchmx)
     dF_build = float(max(np.abs(Jmx - J64).max(), np.abs(Kmx - K64).max()))
     dE_build = abs(float(np.sum(Da.reshape(-1, order="F") * ((Jmx - Kmx) - (J64 - K64)))))
     b64 = args.steps / t64
@@ -415,6 +480,7 @@ def main():
 
     # dominant kernel group of the headline (mixed) build
     roof = roofline_of(stmx, cm64, cm32)
+    roof["timing_launch"] = "one class at a time, 2 blocks/CU, runs of 2048 (the --profile-builds configuration)"
     per_class = {CLASS_NAMES[i]: [round(cm64[i], 3), round(cm32[i], 3)] for i in range(21) if cm64[i] + cm32[i] > 0}
 
     stored = stored_run(L, basis, eng, bufs, Q, args.steps, args.warmup, J64, K64, Da, scf_kw)

@@ -3,7 +3,8 @@
 Translation units:
   csrc/engine.hip             host engine, screening / one-electron / symmetrization kernels
   csrc/k_grad.hip             RHF nuclear-gradient kernels (generic two-electron, one-electron, reduction)
-  csrc/k_grad_eri.hip (many)  one object per light quartet class: the unrolled two-electron gradient
+  csrc/k_grad_eri.hip (many)  the unrolled two-electron gradient: one object per light quartet class,
+                              several (chunks of bra components) per heavier one
   csrc/k_jket.hip (many)      one object per quartet class: the ket-major Coulomb pass of stored builds
   csrc/hfx_scf_gpu.hip        the SCF iteration on the device (rocBLAS/rocSOLVER loaded on first use)
   csrc/k_eri.hip (many)       one object per (quartet class, chunk range): the class kernels
@@ -12,7 +13,8 @@ Translation units:
   tests/c_caller/hfx_c_caller a C++ test program calling the library through include/hfx.h (g++)
 
 Usage: python build.py [-j N] [--only engine hfx_scf ...]
-  --only recompiles just the named objects and links them with the existing others.
+  --only recompiles just the named objects (a name prefixes the objects it selects; with a
+  trailing '=', as in k_grad=, it selects that object alone) and links them with the existing others.
 """
 import argparse
 import concurrent.futures as cf
@@ -37,7 +39,13 @@ CXXFLAGS = ["-O3", "-std=c++20", "-fPIC", "-fopenmp", "-I" + INC, "-Wall"]
 ACC_MAX = 36          # must match k_eri.hip
 CHUNKS_PER_TU = 4     # chunks per translation unit (one launch each; FP64, FP32 and stored-digest kernels,
                       # each with FP64-atomic and fixed-point accumulation)
-GRAD_LMAX_UNROLLED = 3  # gradient classes compiled unrolled (k_grad_eri.hip), one unit each
+GRAD_LMAX_UNROLLED = 3  # gradient classes compiled unrolled (k_grad_eri.hip) with one thread per quartet
+# heavier gradient classes compiled unrolled in chunks of bra components, FP64 only (combo:
+# components per thread, the most that keeps the kernel free of scratch spills; dpdp and ddpp
+# spill a little even with one); dddp and dddd use the generic kernel of k_grad.hip (unrolled,
+# dddp took 2.5x less time on the device but 90 minutes of compiler time)
+GRAD_CHUNKED = {5: 1, 8: 1, 9: 1, 11: 3, 12: 1, 13: 1, 14: 1, 15: 6, 16: 2, 17: 1, 18: 1}
+GRAD_CHUNKS_PER_TU = 3
 CHUNKS_PER_TU_HEAVY = 2  # the same for the classes with la+lb+lc+ld >= 6, whose units compile longest
 
 
@@ -80,6 +88,16 @@ def jobs():
         if la + lb + lc + ld <= GRAD_LMAX_UNROLLED:
             srcs.append(("k_grad_eri_%02d" % k, [HIPCC] + HIPFLAGS + ["-DHFX_GRAD_COMBO=%d" % k, "-c",
                                                                       os.path.join(CSRC, "k_grad_eri.hip")]))
+        elif k in GRAD_CHUNKED:
+            cb = GRAD_CHUNKED[k]
+            nch = (ncart(la) * ncart(lb) + cb - 1) // cb
+            for b in range(0, nch, GRAD_CHUNKS_PER_TU):
+                srcs.append(("k_grad_eri_%02d_%02d" % (k, b),
+                             [HIPCC] + HIPFLAGS + ["-DHFX_GRAD_COMBO=%d" % k, "-DHFX_GRAD_FP64_ONLY",
+                                                   "-DHFX_GRAD_CB=%d" % cb,
+                                                   "-DHFX_GRAD_CHUNK_BEGIN=%d" % b,
+                                                   "-DHFX_GRAD_CHUNKS=%d" % min(GRAD_CHUNKS_PER_TU, nch - b),
+                                                   "-c", os.path.join(CSRC, "k_grad_eri.hip")]))
     for (k, la, lb, lc, ld, nch) in combo_classes():
         step = CHUNKS_PER_TU_HEAVY if la + lb + lc + ld >= 6 else CHUNKS_PER_TU
         for b in range(0, nch, step):
@@ -106,11 +124,14 @@ def build(jobs_n=None, verbose=False, only=None):
     todo = jobs()
     keep = []
     if only:
-        keep = [os.path.join(BUILD, name + ".o") for name, _ in todo if not name.startswith(tuple(only))]
+        # a name matches its object and, unless it ends with '=', every object it prefixes
+        def sel(name):
+            return any(name == o[:-1] if o.endswith("=") else name.startswith(o) for o in only)
+        keep = [os.path.join(BUILD, name + ".o") for name, _ in todo if not sel(name)]
         missing = [o for o in keep if not os.path.exists(o)]
         if missing:
             raise RuntimeError("--only needs a full build first; missing " + ", ".join(missing[:4]))
-        todo = [j for j in todo if j[0].startswith(tuple(only))]
+        todo = [j for j in todo if sel(j[0])]
     n = jobs_n or min(16, os.cpu_count() or 8)
     objs, failed = [], []
     t0 = time.time()

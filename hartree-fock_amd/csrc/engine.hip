@@ -428,6 +428,9 @@ __global__ void k_ranges(const unsigned long long* offs, const long long* slots,
     ranges[2 * k + 1] = (long long)offs[slots[3 * k + 1]];
     ranges[2 * N_COMBOS + 2 * k] = (long long)offs[slots[3 * k + 1]];
     ranges[2 * N_COMBOS + 2 * k + 1] = (long long)offs[slots[3 * k + 2]];
+    // both parts of the class (they are adjacent in the list)
+    ranges[4 * N_COMBOS + 2 * k] = (long long)offs[slots[3 * k]];
+    ranges[4 * N_COMBOS + 2 * k + 1] = (long long)offs[slots[3 * k + 2]];
 }
 
 // out = sum over the accumulator copies of A + A^T, within every molecule block
@@ -805,7 +808,7 @@ struct hfx_engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     BuildParams graph_key{-1.0, -1.0, -1.0, false, false, false};
-    long long* d_ranges = nullptr;          // [4*N_COMBOS]
+    long long* d_ranges = nullptr;          // [6*N_COMBOS]: FP64 parts, FP32 parts, both
     ScreenClass* d_screen_cls = nullptr;     // [N_COMBOS]: class lists for the screening kernels
     int screen_nbra[N_COMBOS] = {};          // their bra counts (0: class without work)
     unsigned int* d_chunk_ctr = nullptr;     // [2*N_COMBOS*CHUNK_SLOTS]: chunk counters per class kernel
@@ -1373,7 +1376,7 @@ int create_impl(int n_mol, const hfx_basis* bases, int device, hfx_engine** out)
                                             (int)(e->n_slots + 1), e->stream));
     e->d_scan_tmp = e->mem.alloc<char>(e->scan_tmp_bytes);
     e->d_stats = e->mem.alloc<unsigned long long>(6 * N_COMBOS);
-    e->d_ranges = e->mem.alloc<long long>(4 * N_COMBOS);
+    e->d_ranges = e->mem.alloc<long long>(6 * N_COMBOS);
     e->d_chunk_ctr = e->mem.alloc<unsigned int>(2 * N_COMBOS * hfx_engine::CHUNK_SLOTS);
     for (int m = 0; m < n_mol; ++m) e->max_N = std::max(e->max_N, e->mol_N[m]);
     if (!e->d_Tt || !e->d_Ta || !e->d_Tb || !e->d_Dt || !e->d_Da || !e->d_Db || !e->d_Mt || !e->d_Ma || !e->d_Mb || !e->d_AJ || !e->d_AKa || !e->d_AKb ||
@@ -2155,9 +2158,10 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         if (!(e->cls_ids[p1].empty() || e->cls_ids[p2].empty())) {
             const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
-            g.range = e->d_ranges + 2 * k;
+            const bool split = mixed && grad_class_has_fp32(la, lb, lc, ld);
+            g.range = e->d_ranges + (mixed && !split ? 4 * N_COMBOS : 0) + 2 * k;
             launch_grad_eri(0, g, la, lb, lc, ld, grid, s);
-            if (mixed) {
+            if (split) {
                 g.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
                 launch_grad_eri(1, g, la, lb, lc, ld, grid, s);
             }
@@ -2170,8 +2174,10 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
     HIPCHK(hipGetLastError());
     std::vector<double> out(3 * (size_t)n3);
     long long rng[4 * N_COMBOS];
+    unsigned long long stv[6 * N_COMBOS];
     HIPCHK(hipMemcpyAsync(out.data(), e->d_gout, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(rng, e->d_ranges, sizeof(rng), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(stv, e->d_stats, sizeof(stv), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     // nuclear repulsion: d/dX_A sum_{A<B} Z_A Z_B / R_AB
     std::vector<double> gn(n3, 0.0);
@@ -2197,7 +2203,9 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
             st->n_quartets += rng[2 * k + 1] - rng[2 * k];
             const long long n32 = rng[2 * N_COMBOS + 2 * k + 1] - rng[2 * N_COMBOS + 2 * k];
             st->n_quartets += n32;
-            st->n_fp32 += n32;
+            const int p1 = combo_pc1(k), p2 = combo_pc2(k);
+            if (grad_class_has_fp32(pc_la(p1), pc_lb(p1), pc_la(p2), pc_lb(p2))) st->n_fp32 += n32;
+            st->class_prims[k] = (int64_t)(stv[6 * k + 2] + stv[6 * k + 3]);  // both parts, unpruned pair lists
         }
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));

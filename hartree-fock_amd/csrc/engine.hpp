@@ -142,10 +142,14 @@ struct GradArgs {
 };
 void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, hipStream_t s);
 // The two-electron gradient kernels of one quartet class, compiled with every angular loop
-// unrolled (k_grad_eri.hip, one translation unit per class); classes without one use the
-// generic kernel of k_grad.hip.
+// unrolled (k_grad_eri.hip, one or more translation units per class, each for a range of
+// bra-component chunks starting at first_chunk; fp32: the unit has FP32 kernels too); classes
+// without one use the generic kernel of k_grad.hip.
 using GradClassFn = void (*)(int fp32, const GradArgs& g, int grid, hipStream_t s);
-bool register_grad_class(int combo, GradClassFn fn);
+bool register_grad_class(int combo, int first_chunk, GradClassFn fn, bool fp32);
+// A mixed-precision gradient runs an FP32 pass for the class only where this holds (unrolled
+// FP32 kernels); elsewhere the class's whole list goes through the FP64 kernels in one pass.
+bool grad_class_has_fp32(int la, int lb, int lc, int ld);
 void launch_grad_one(const GradArgs& g, int grid, hipStream_t s);
 void launch_grad_reduce(const GradArgs& g, int n_rows, double* out, hipStream_t s);
 

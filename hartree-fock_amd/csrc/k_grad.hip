@@ -11,11 +11,12 @@
 // nuclear attraction is dR_tuv(P - C)/dC_x = -R_{t+1,u,v}.  Integrals are recomputed, never
 // stored.
 //
-// Work layout: one 64-lane wave per block.  The two-electron kernel takes one screened
-// shell quartet of the J/K work list at a time (grid-stride); the wave walks the primitive
-// quartets together, builds the Boys values, R (in LDS, in place, one angular shell per
-// step) and the E tables (in LDS), and the lanes split the Cartesian component quartets.
-// The one-electron kernel takes one shell pair at a time the same way.  Each block adds its
+// Work layout: one block per work item.  The two-electron kernel (four wavefronts) takes one
+// screened shell quartet of the J/K work list at a time (grid-stride); the block walks the
+// primitive quartets together, builds the Boys values, R (in LDS, in place, one angular shell
+// per step) and the E tables (in LDS), and the threads split the Cartesian component quartets.
+// The one-electron kernel (one wavefront) takes one (shell pair, operator) item at a time the same way: the
+// overlap and kinetic terms of the pair, or the attraction of one nucleus.  Each block adds its
 // results into its own row of a partial-gradient array (no atomics); k_grad_reduce sums the
 // rows.
 #include "engine.hpp"
@@ -83,7 +84,7 @@ __device__ R boys_one(int n, R T, const R* __restrict__ tab) {
 
 // R^0_tuv for t + u + v <= M into X (cube of edge RD), from Fs[n] = (-2 alpha)^n F_n:
 // levels n = M-1 .. 0 are built in place, shells s = M-n .. 1 top down, so that every
-// source (shells s-1, s-2) still holds level n+1.  Called by the whole wave.
+// source (shells s-1, s-2) still holds level n+1.  Called by the whole block.
 template <class R>
 __device__ void build_r(R* X, const R* Fs, int M, R px, R py, R pz) {
     const int lane = threadIdx.x;
@@ -151,14 +152,17 @@ struct GradLists<float> {
 // Two-electron part: 1/2 deg sum_{block} Gamma d(ab|cd)/dX for every screened quartet of
 // one class; centres A, B, C explicitly, D = -(A + B + C) by translational invariance.
 // ---------------------------------------------------------------------------------------
+constexpr int GEN_T = 256;  // threads of the generic two-electron kernel's block (four wavefronts)
+
 template <class R>
-__global__ __launch_bounds__(64) void k_grad_eri(GradArgs g, int la, int lb, int lc, int ld) {
+__global__ __launch_bounds__(GEN_T) void k_grad_eri(GradArgs g, int la, int lb, int lc, int ld) {
     using L = GradLists<R>;
     __shared__ R X[RD * RD * RD];
     __shared__ R Eb[3 * EI * EI * ET];
     __shared__ R Ek[3 * EI * EI * ET];
     __shared__ double Gm[1296];
     __shared__ R Fs[16];
+    __shared__ double red[GEN_T / 64][9];
     const int lane = threadIdx.x;
     const long long lo = g.range[0], hi = g.range[1];
     const int nA = md::ncart(la), nB = md::ncart(lb), nC = md::ncart(lc), nD = md::ncart(ld);
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(64) void k_grad_eri(GradArgs g, int la, int lb, int
         const int ia = g.sh_ao[sa], ib = g.sh_ao[sb], ic = g.sh_ao[sc], id = g.sh_ao[sd];
         const int N = g.n_ao;
         // Gamma of the block, with the Cartesian normalization ratios and 1/2 deg folded in
-        for (int c = lane; c < ncomp; c += 64) {
+        for (int c = lane; c < ncomp; c += GEN_T) {
             const int D = c % nD, C = (c / nD) % nC, B = (c / (nD * nC)) % nB, A = c / (nD * nC * nB);
             const size_t i = ia + A, j = ib + B, k = ic + C, l = id + D;
             auto P = [&](size_t r, size_t s) { return g.P[r + s * N]; };
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(64) void k_grad_eri(GradArgs g, int la, int lb, int
                 }
                 __syncthreads();
                 build_r<R>(X, Fs, M, px, py, pz);
-                for (int c = lane; c < ncomp; c += 64) {
+                for (int c = lane; c < ncomp; c += GEN_T) {
                     const R w = R(Gm[c]) * pref;
                     if (w == R(0)) continue;
                     const int D = c % nD, C = (c / nD) % nC, B = (c / (nD * nC)) % nB, A = c / (nD * nC * nB);
@@ -309,7 +313,12 @@ __global__ __launch_bounds__(64) void k_grad_eri(GradArgs g, int la, int lb, int
             v[3 + d] = wave_sum(gB[d]);
             v[6 + d] = wave_sum(gC[d]);
         }
+        if ((lane & 63) == 0)
+            for (int x = 0; x < 9; ++x) red[lane >> 6][x] = v[x];
+        __syncthreads();
         if (lane == 0) {
+            for (int w = 1; w < GEN_T / 64; ++w)
+                for (int x = 0; x < 9; ++x) v[x] += red[w][x];
             const int aa = g.sh_atom[sa], ab = g.sh_atom[sb], ac = g.sh_atom[sc], ad = g.sh_atom[sd];
             for (int d = 0; d < 3; ++d) {
                 row[3 * aa + d] += v[d];
@@ -318,7 +327,7 @@ __global__ __launch_bounds__(64) void k_grad_eri(GradArgs g, int la, int lb, int
                 row[3 * ad + d] -= v[d] + v[3 + d] + v[6 + d];
             }
         }
-        __syncthreads();  // Gm is rewritten by the next quartet
+        __syncthreads();  // Gm and red are rewritten by the next quartet
     }
 }
 
@@ -335,7 +344,12 @@ __global__ __launch_bounds__(64) void k_grad_one(GradArgs g) {
     double* rowS = g.part + (size_t)blockIdx.x * g.part_stride;
     double* rowH = rowS + g.n3;
     const int N = g.n_ao;
-    for (int pid = blockIdx.x; pid < g.n_pairs; pid += gridDim.x) {
+    // one work item per (shell pair, operator): item % (n_atoms + 1) = 0 is the overlap and
+    // kinetic terms, 1 + C the attraction of nucleus C
+    const long long n_items = (long long)g.n_pairs * (g.n_atoms + 1);
+    for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int pid = (int)(item / (g.n_atoms + 1));
+        const int C = (int)(item % (g.n_atoms + 1)) - 1;
         const int sa = g.pair_a[pid], sb = g.pair_b[pid];
         const int la = g.sh_l[sa], lb = g.sh_l[sb];
         const int nA = md::ncart(la), nB = md::ncart(lb), ncomp = nA * nB;
@@ -346,8 +360,7 @@ __global__ __launch_bounds__(64) void k_grad_one(GradArgs g) {
         const int ia = g.sh_ao[sa], ib = g.sh_ao[sb];
         double gSA[3] = {0, 0, 0}, gSB[3] = {0, 0, 0}, gHA[3] = {0, 0, 0}, gHB[3] = {0, 0, 0};
         const int M = la + lb + 1;
-        // nucleus index -1: overlap and kinetic terms; 0..n_atoms-1: nuclear attraction
-        for (int C = -1; C < g.n_atoms; ++C) {
+        {
             double gCC[3] = {0, 0, 0};
             for (int x1 = 0; x1 < g.sh_np[sa]; ++x1)
                 for (int x2 = 0; x2 < g.sh_np[sb]; ++x2) {
@@ -470,12 +483,15 @@ __global__ __launch_bounds__(64) void k_grad_one(GradArgs g) {
         if (lane == 0) {
             const int aa = g.sh_atom[sa], ab = g.sh_atom[sb];
             for (int d = 0; d < 3; ++d) {
-                rowS[3 * aa + d] += v[d];
-                rowS[3 * ab + d] += v[3 + d];
+                if (C < 0) {
+                    rowS[3 * aa + d] += v[d];
+                    rowS[3 * ab + d] += v[3 + d];
+                }
                 rowH[3 * aa + d] += v[6 + d];
                 rowH[3 * ab + d] += v[9 + d];
             }
         }
+        __syncthreads();  // E, X and Fs are rewritten by the next item
     }
 }
 
@@ -490,29 +506,54 @@ __global__ void k_grad_reduce(const double* __restrict__ part, long long stride,
 
 }  // namespace
 
-// Launchers of the compile-time-unrolled classes (k_grad_eri.hip, one translation unit per
-// class), registered at load time.
-GradClassFn& grad_class_fn(int combo) {
-    static GradClassFn table[N_COMBOS] = {};
+// Launchers of the compile-time-unrolled classes (k_grad_eri.hip: one or more translation
+// units per class, each for a range of bra-component chunks), registered at load time and
+// called in the order of their first chunk.
+namespace {
+constexpr int GRAD_UNITS_MAX = 64;
+struct GradUnits {
+    int n = 0;
+    bool fp32 = true;  // every unit of the class has FP32 kernels
+    int first[GRAD_UNITS_MAX];
+    GradClassFn fn[GRAD_UNITS_MAX];
+};
+GradUnits& grad_units(int combo) {
+    static GradUnits table[N_COMBOS];
     return table[combo];
 }
+}  // namespace
 
-bool register_grad_class(int combo, GradClassFn fn) {
+bool register_grad_class(int combo, int first_chunk, GradClassFn fn, bool fp32) {
     if (combo < 0 || combo >= N_COMBOS) return false;
-    grad_class_fn(combo) = fn;
+    GradUnits& u = grad_units(combo);
+    if (u.n == GRAD_UNITS_MAX) return false;
+    u.fp32 = u.fp32 && fp32;
+    int i = u.n++;
+    for (; i > 0 && u.first[i - 1] > first_chunk; --i) {
+        u.first[i] = u.first[i - 1];
+        u.fn[i] = u.fn[i - 1];
+    }
+    u.first[i] = first_chunk;
+    u.fn[i] = fn;
     return true;
+}
+
+bool grad_class_has_fp32(int la, int lb, int lc, int ld) {
+    const GradUnits& u = grad_units(combo_of(pair_class(la, lb), pair_class(lc, ld)));
+    return u.n > 0 && u.fp32;
 }
 
 void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, hipStream_t s) {
     const int k = combo_of(pair_class(la, lb), pair_class(lc, ld));
-    if (GradClassFn fn = grad_class_fn(k)) {
-        fn(fp32, g, grid, s);
+    const GradUnits& u = grad_units(k);
+    if (u.n > 0) {
+        for (int i = 0; i < u.n; ++i) u.fn[i](fp32, g, grid, s);
         return;
     }
     if (fp32)
-        hipLaunchKernelGGL(k_grad_eri<float>, dim3(grid), dim3(64), 0, s, g, la, lb, lc, ld);
+        hipLaunchKernelGGL(k_grad_eri<float>, dim3(grid), dim3(GEN_T), 0, s, g, la, lb, lc, ld);
     else
-        hipLaunchKernelGGL(k_grad_eri<double>, dim3(grid), dim3(64), 0, s, g, la, lb, lc, ld);
+        hipLaunchKernelGGL(k_grad_eri<double>, dim3(grid), dim3(GEN_T), 0, s, g, la, lb, lc, ld);
 }
 
 void launch_grad_one(const GradArgs& g, int grid, hipStream_t s) {

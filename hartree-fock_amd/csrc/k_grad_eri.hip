@@ -1,14 +1,21 @@
 // k_grad_eri.hip - two-electron part of the RHF nuclear gradient for one quartet class, every
 // angular loop unrolled at compile time (the pattern of the J/K kernels of k_eri.hip).
 //
-// Compiled once per class with -DHFX_GRAD_COMBO=k (k as in k_eri.hip) for the classes with
-// la+lb+lc+ld <= 3; the heavier classes (whose unrolled form spills to scratch) use the generic
-// kernel of k_grad.hip.
+// Compiled per class with -DHFX_GRAD_COMBO=k (k as in k_eri.hip).  Up to la+lb+lc+ld = 3 one
+// thread takes a whole quartet.  Above that the whole quartet spills to scratch, so the bra
+// components are split into chunks of HFX_GRAD_CB: one kernel per chunk, each thread takes
+// one (quartet, chunk) and repeats the Boys values, R and the E tables for its chunk.  A
+// translation unit holds the chunks HFX_GRAD_CHUNK_BEGIN .. + HFX_GRAD_CHUNKS - 1 of its class
+// and launches them in order on the caller's stream; the classes not compiled here use the
+// generic kernel of k_grad.hip.  The chunked classes are compiled in FP64 only
+// (-DHFX_GRAD_FP64_ONLY): a mixed-precision gradient runs their whole list, the quartets below
+// the cut included, through the FP64 kernels in one pass (a second, FP32 pass over the short
+// tail of each chunk took longer than the FP64 work it replaced), and the compile time halves.
 //
-// One thread per screened quartet of the J/K work list built from (P, P/2).  The thread writes
-// the quartet's
+// One thread per screened quartet (and chunk) of the J/K work list built from (P, P/2).  The
+// thread writes the part of the quartet's
 //   Gamma_{ab,cd} = 1/2 deg [P_ij P_kl - 1/4 (P_ik P_jl + P_il P_jk)] N_a N_b N_c N_d
-// into its own LDS slots, then walks the primitive quartets: F_0..F_{L+1} (L = la+lb+lc+ld),
+// that belongs to its chunk into its own LDS slots, then walks the primitive quartets: F_0..F_{L+1} (L = la+lb+lc+ld),
 // R^0_tuv for t+u+v <= L+1 scaled by the prefactor, E of the bra up to (la+1, lb) and of the
 // ket up to (lc+1, ld).  For every component pair (ab, cd) it forms
 //   Y[t,u,v]   = sum_{t'u'v'} (-1)^{t'+u'+v'} E^{cd}_{t'u'v'} R_{t+t',u+u',v+v'}
@@ -38,7 +45,19 @@ constexpr int LA = pc_la(PC1), LB = pc_lb(PC1), LC = pc_la(PC2), LD = pc_lb(PC2)
 constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
 constexpr int NAB = NA * NB, NCD = NC * ND;
 constexpr int LT = LA + LB + LC + LD + 1;  // highest Boys order and R level
-static_assert(LT <= 4, "unrolled gradient classes end at la+lb+lc+ld = 3");
+#ifndef HFX_GRAD_CB
+#define HFX_GRAD_CB NAB
+#endif
+#ifndef HFX_GRAD_CHUNK_BEGIN
+#define HFX_GRAD_CHUNK_BEGIN 0
+#endif
+#ifndef HFX_GRAD_CHUNKS
+#define HFX_GRAD_CHUNKS 1
+#endif
+constexpr int CB = HFX_GRAD_CB;  // bra components per thread
+constexpr int CH_BEGIN = HFX_GRAD_CHUNK_BEGIN, CH_COUNT = HFX_GRAD_CHUNKS;
+static_assert(CB >= 1 && CB <= NAB && CH_COUNT >= 1 && (CH_BEGIN + CH_COUNT - 1) * CB < NAB,
+              "chunks of bra components outside the class");
 // Threads per block: the Gamma slots of every thread fit in 40 KB of LDS.
 constexpr int block_threads(int bytes) { return bytes * 256 <= 40960 ? 256 : (bytes * 128 <= 40960 ? 128 : 64); }
 constexpr int GRAD_LDS_MAX = 4096;  // gradient entries (3 per atom) a block keeps in LDS
@@ -78,11 +97,13 @@ __device__ __forceinline__ R ed(const Ecoef<EA, EB, R>& E, R twoa) {
     return v;
 }
 
-template <class R>
-__global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_grad_eri_t(GradArgs g) {
-    constexpr int BT = block_threads(NAB * NCD * (int)sizeof(R));
+// KC: the class (K_COMBO; it names the kernel in profiles), C0: the chunk's first bra
+// component (components C0 .. C0 + CB - 1, those below NAB).
+template <class R, int KC, int C0>
+__global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_grad_eri_t(GradArgs g) {
+    constexpr int BT = block_threads(CB * NCD * (int)sizeof(R));
     using Ls = Lists<R>;
-    __shared__ R gam[NAB * NCD * BT];
+    __shared__ R gam[CB * NCD * BT];
     extern __shared__ double sgrad[];
     const int n3 = g.n3;
     const bool lds_rows = n3 <= GRAD_LDS_MAX;
@@ -103,8 +124,10 @@ __global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_g
         const double deg = (sa != sb ? 2.0 : 1.0) * (sc != sd ? 2.0 : 1.0) * (p1 != p2 ? 2.0 : 1.0);
         {
             const int oA = g.sh_ao[sa], oB = g.sh_ao[sb], oC = g.sh_ao[sc], oD = g.sh_ao[sd];
-            sfor<NAB>([&](auto ab_) {
-                constexpr int ab = decltype(ab_)::value;
+            sfor<CB>([&](auto c_) {
+                constexpr int c = decltype(c_)::value;
+                constexpr int ab = C0 + c;
+                if constexpr (ab < NAB) {
                 constexpr int A = ab / NB, B = ab % NB;
                 const size_t i = oA + A, j = oB + B;
                 const double cab = 0.5 * deg * g.sh_cf[6 * sa + A] * g.sh_cf[6 * sb + B];
@@ -115,8 +138,9 @@ __global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_g
                     const size_t k = oC + C, l = oD + D;
                     const double gmm =
                         pij * g.P[k + l * N] - 0.25 * (g.P[i + k * N] * g.P[j + l * N] + g.P[i + l * N] * g.P[j + k * N]);
-                    gm[(ab * NCD + cd) * BT] = (R)(gmm * cab * g.sh_cf[6 * sc + C] * g.sh_cf[6 * sd + D]);
+                    gm[(c * NCD + cd) * BT] = (R)(gmm * cab * g.sh_cf[6 * sc + C] * g.sh_cf[6 * sd + D]);
                 });
+                }
             });
         }
         double gA[3] = {0, 0, 0}, gP[3] = {0, 0, 0}, gC[3] = {0, 0, 0};
@@ -156,8 +180,10 @@ __global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_g
                 }
                 R Rt[nherm(LT)];
                 hermite_r<LT>(base, X, Y, Z, Rt);
-                sfor<NAB>([&](auto ab_) {
-                    constexpr int ab = decltype(ab_)::value;
+                sfor<CB>([&](auto c_) {
+                    constexpr int c = decltype(c_)::value;
+                    constexpr int ab = C0 + c;
+                    if constexpr (ab < NAB) {
                     constexpr int A = ab / NB, B = ab % NB;
                     constexpr int ax = cart_i(LA, A), ay = cart_j(LA, A), az = cart_k(LA, A);
                     constexpr int bx = cart_i(LB, B), by = cart_j(LB, B), bz = cart_k(LB, B);
@@ -168,7 +194,7 @@ __global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_g
                         constexpr int cx = cart_i(LC, C), cy = cart_j(LC, C), cz = cart_k(LC, C);
                         constexpr int dx = cart_i(LD, D), dy = cart_j(LD, D), dz = cart_k(LD, D);
                         constexpr int kx = cx + dx, ky = cy + dy, kz = cz + dz;
-                        const R w = gm[(ab * NCD + cd) * BT];
+                        const R w = gm[(c * NCD + cd) * BT];
                         // Y over the bra box extended by one in at most one direction
                         R Yv[lx + 2][ly + 2][lz + 2];
                         sfor<lz + 2>([&](auto v_) {
@@ -281,6 +307,7 @@ __global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_g
                         // register peak
                         __builtin_amdgcn_sched_barrier(0);
                     });
+                    }
                 });
             }
             sfor<3>([&](auto d_) {
@@ -306,15 +333,34 @@ __global__ __launch_bounds__(block_threads(NAB * NCD * (int)sizeof(R))) void k_g
     }
 }
 
-void launch(int fp32, const GradArgs& g, int grid, hipStream_t s) {
-    const size_t lds = g.n3 <= GRAD_LDS_MAX ? (size_t)g.n3 * sizeof(double) : 0;
-    if (fp32)
-        hipLaunchKernelGGL(k_grad_eri_t<float>, dim3(grid), dim3(block_threads(NAB * NCD * 4)), lds, s, g);
-    else
-        hipLaunchKernelGGL(k_grad_eri_t<double>, dim3(grid), dim3(block_threads(NAB * NCD * 8)), lds, s, g);
+// The chunks of this unit, one after the other on the caller's stream (every kernel adds into
+// the same per-block rows of g.part, so they must not overlap).
+template <int I>
+void launch_chunks(int fp32, const GradArgs& g, int grid, hipStream_t s, size_t lds) {
+    if constexpr (I < CH_COUNT) {
+        constexpr int c0 = (CH_BEGIN + I) * CB;
+#ifdef HFX_GRAD_FP64_ONLY
+        (void)fp32;
+#else
+        if (fp32)
+            hipLaunchKernelGGL((k_grad_eri_t<float, K_COMBO, c0>), dim3(grid), dim3(block_threads(CB * NCD * 4)), lds, s, g);
+        else
+#endif
+            hipLaunchKernelGGL((k_grad_eri_t<double, K_COMBO, c0>), dim3(grid), dim3(block_threads(CB * NCD * 8)), lds, s,
+                               g);
+        launch_chunks<I + 1>(fp32, g, grid, s, lds);
+    }
 }
 
-const bool registered = register_grad_class(K_COMBO, &launch);
+void launch(int fp32, const GradArgs& g, int grid, hipStream_t s) {
+    launch_chunks<0>(fp32, g, grid, s, g.n3 <= GRAD_LDS_MAX ? (size_t)g.n3 * sizeof(double) : 0);
+}
+
+#ifdef HFX_GRAD_FP64_ONLY
+const bool registered = register_grad_class(K_COMBO, CH_BEGIN, &launch, false);
+#else
+const bool registered = register_grad_class(K_COMBO, CH_BEGIN, &launch, true);
+#endif
 
 }  // namespace
 }  // namespace hfx

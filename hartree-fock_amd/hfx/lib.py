@@ -75,7 +75,7 @@ class HfxGradArgs(C.Structure):
 
 class HfxGradStats(C.Structure):
     _fields_ = [("n_quartets", C.c_int64), ("n_fp32", C.c_int64), ("device_ms", C.c_double),
-                ("class_ms", C.c_double * 21)]
+                ("class_ms", C.c_double * 21), ("class_prims", C.c_int64 * 21)]
 
 
 class HfxScfOptions(C.Structure):
@@ -342,7 +342,8 @@ def rhf_gradient(engine, basis, P, W, schwarz_thresh=1e-10, density_thresh=1e-10
     engine.lib.check(engine.lib.dll.hfx_rhf_gradient(engine.h, C.byref(a), _p(g), _p(parts), C.byref(st)),
                      "hfx_rhf_gradient")
     return g.reshape(nat, 3), parts.reshape(4, nat, 3), dict(n_quartets=st.n_quartets, n_fp32=st.n_fp32,
-                                                            device_ms=st.device_ms, class_ms=np.array(st.class_ms))
+                                                            device_ms=st.device_ms, class_ms=np.array(st.class_ms),
+                                                            class_prims=np.array(st.class_prims))
 
 
 def energy_weighted_density(C, eps, n_occ):

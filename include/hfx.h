@@ -229,7 +229,9 @@ typedef struct hfx_grad_args {
     const int32_t* shell_atom; /* atom of every shell, or NULL to match shell centres       */
     double schwarz_thresh;     /* quartets with Q(ab)Q(cd) < tau_S are skipped               */
     double density_thresh;     /* and those failing the hfx_jk density test on (P, P/2)      */
-    int32_t precision;         /* HFX_FP64, or HFX_MIXED with fp32_cut as in hfx_jk          */
+    int32_t precision;         /* HFX_FP64, or HFX_MIXED with fp32_cut as in hfx_jk; the
+                                  FP32 part covers the classes with la+lb+lc+ld <= 3, the
+                                  heavier ones run in FP64 throughout                       */
     double fp32_cut;
     int32_t flags;             /* HFX_JK_DEVICE_PTRS: P and W are device pointers;
                                   HFX_JK_CLASS_TIMING: per-class times in the stats        */
@@ -237,10 +239,11 @@ typedef struct hfx_grad_args {
 
 typedef struct hfx_grad_stats {
     int64_t n_quartets;        /* shell quartets differentiated                            */
-    int64_t n_fp32;
+    int64_t n_fp32;            /* of which differentiated in FP32 (MIXED)                  */
     double device_ms;          /* device time of the two-electron part                     */
     double class_ms[21];       /* with HFX_JK_CLASS_TIMING in the flags: device time of each
                                   quartet class (both precisions), in hfx_jk_stats' order   */
+    int64_t class_prims[21];   /* primitive quartets differentiated in each class          */
 } hfx_grad_stats;
 
 /* grad[3*n_atoms] (x,y,z per atom).  parts, when not NULL, receives 4 x 3*n_atoms:

@@ -86,6 +86,8 @@ def test_gradient_mixed_precision(engine_lib):
     gmx, _, smx = lib.rhf_gradient(e, b, P, W, precision=lib.HFX_MIXED, fp32_cut=1e-5)
     assert 0 < smx["n_fp32"] < smx["n_quartets"] == s64["n_quartets"]
     assert np.abs(gmx - g64).max() <= 1e-6
+    # the same screened quartets, every one with at least one primitive quartet
+    assert (smx["class_prims"] == s64["class_prims"]).all() and s64["class_prims"].sum() >= s64["n_quartets"]
 
 
 def test_autograd_function():
