@@ -136,7 +136,8 @@ def build(jobs_n=None, verbose=False, only=None):
     objs, failed = [], []
     t0 = time.time()
     # the biggest classes first, so the long compiles start early
-    order = sorted(todo, key=lambda j: (not j[0].startswith("k_eri_2"), not j[0].startswith("k_eri_1"), j[0]))
+    first = ("k_eri_2", "k_grad_eri_1", "k_eri_1")
+    order = sorted(todo, key=lambda j: (next((i for i, f in enumerate(first) if j[0].startswith(f)), len(first)), j[0]))
     with cf.ThreadPoolExecutor(max_workers=n) as ex:
         futs = [ex.submit(_run, name, cmd) for name, cmd in order]
         for f in cf.as_completed(futs):
