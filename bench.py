@@ -182,9 +182,10 @@ def grad_model_flop(la, lb, lc, ld):
             eri_model_flop(la, lb, lc + 1, ld) + 18.0 * ncomp)
 
 
-def gradient_roofline(st):
-    """Roofline of the gradient's dominant class (its kernels, one class): model FLOPs of its
-    primitive quartets (grad_model_flop) over its device time, against the FP64 vector peak."""
+def gradient_roofline(st, device_ms):
+    """Roofline of the gradient's dominant class (its kernels, one class, timed alone): model FLOPs
+    of its primitive quartets (grad_model_flop) over its device time, against the FP64 vector peak;
+    the whole two-electron part over device_ms (the classes overlapping on four streams)."""
     cms, prims = st["class_ms"], st["class_prims"]
     k = int(np.argmax(cms))
     gfl = float(prims[k]) * grad_model_flop(*CLASS_L[k]) * 1e-9
@@ -196,7 +197,7 @@ def gradient_roofline(st):
     return dict(bound="valu", pipe="VALU f64", kernel="%s %s" % (kern, CLASS_NAMES[k]), achieved=achieved,
                 peak=PEAK_TFLOPS["f64"], unit="TFLOP/s", frac=achieved / PEAK_TFLOPS["f64"], traffic=None,
                 launch_ms=float(cms[k]), launch_model_gflop=gfl, prim_quartets=int(prims[k]),
-                two_electron_model_gflop=total, two_electron_tflops=total / float(np.sum(cms)),
+                two_electron_model_gflop=total, two_electron_tflops=total / device_ms,
                 class_model_tflops=per)
 
 
@@ -220,7 +221,7 @@ def gradient_run(L):
                 quartets=s64["n_quartets"], fp32_quartets=smx["n_fp32"],
                 max_dgrad_mixed=float(np.abs(gmx - g64).max()), max_abs_grad=float(np.abs(g64).max()),
                 class_ms_fp64={CLASS_NAMES[i]: round(float(stc["class_ms"][i]), 3) for i in range(21)},
-                roofline=gradient_roofline(stc))
+                roofline=gradient_roofline(stc, s64["device_ms"]))
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_eri_c20h42_mixed_traffic.json")

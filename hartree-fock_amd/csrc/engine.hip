@@ -2154,19 +2154,34 @@ int grad_impl(hfx_engine* e, const hfx_grad_args* a, double* grad, double* parts
         for (auto& ev : cev) HIPCHK(hipEventCreate(&ev));
     }
     HIPCHK(hipEventRecord(e->ev0, s));
+    // The class kernels go round-robin over the engine's streams (forked from s and joined back
+    // before the one-electron part); with per-class timing they stay on s, one after another.
+    const int n_str = timing ? 1 : e->n_side;
+    hipStream_t ss[hfx_engine::N_SIDE];
+    ss[0] = s;
+    for (int i = 1; i < n_str; ++i) ss[i] = e->side[i];
+    if (n_str > 1) {
+        HIPCHK(hipEventRecord(e->ev_fork, s));
+        for (int i = 1; i < n_str; ++i) HIPCHK(hipStreamWaitEvent(e->side[i], e->ev_fork, 0));
+    }
+    int next = 0;
     for (int k = 0; k < N_COMBOS; ++k) {
         const int p1 = combo_pc1(k), p2 = combo_pc2(k);
         if (!(e->cls_ids[p1].empty() || e->cls_ids[p2].empty())) {
             const int la = pc_la(p1), lb = pc_lb(p1), lc = pc_la(p2), ld = pc_lb(p2);
             const bool split = mixed && grad_class_has_fp32(la, lb, lc, ld);
             g.range = e->d_ranges + (mixed && !split ? 4 * N_COMBOS : 0) + 2 * k;
-            launch_grad_eri(0, g, la, lb, lc, ld, grid, s);
+            launch_grad_eri(0, g, la, lb, lc, ld, grid, ss, n_str, &next);
             if (split) {
                 g.range = e->d_ranges + 2 * N_COMBOS + 2 * k;
-                launch_grad_eri(1, g, la, lb, lc, ld, grid, s);
+                launch_grad_eri(1, g, la, lb, lc, ld, grid, ss, n_str, &next);
             }
         }
         if (timing) HIPCHK(hipEventRecord(cev[k], s));
+    }
+    for (int i = 1; i < n_str; ++i) {
+        HIPCHK(hipEventRecord(e->ev_join[i], e->side[i]));
+        HIPCHK(hipStreamWaitEvent(s, e->ev_join[i], 0));
     }
     HIPCHK(hipEventRecord(e->ev1, s));
     launch_grad_one(g, grid, s);

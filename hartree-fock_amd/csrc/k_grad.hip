@@ -167,6 +167,10 @@ __global__ __launch_bounds__(GEN_T) void k_grad_eri(GradArgs g, int la, int lb, 
     const long long lo = g.range[0], hi = g.range[1];
     const int nA = md::ncart(la), nB = md::ncart(lb), nC = md::ncart(lc), nD = md::ncart(ld);
     const int ncomp = nA * nB * nC * nD;
+    // blockIdx.y: this block's slice of the component quartets (a class with few quartets and
+    // many components fills the machine only with several blocks per quartet)
+    const int per = (ncomp + gridDim.y - 1) / gridDim.y;
+    const int c_lo = blockIdx.y * per, c_hi = min(ncomp, c_lo + per);
     const int M = la + lb + lc + ld + 1;
     double* row = g.part + (size_t)blockIdx.x * g.part_stride + 2 * g.n3;
     for (long long q = lo + blockIdx.x; q < hi; q += gridDim.x) {
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(GEN_T) void k_grad_eri(GradArgs g, int la, int lb, 
         const int ia = g.sh_ao[sa], ib = g.sh_ao[sb], ic = g.sh_ao[sc], id = g.sh_ao[sd];
         const int N = g.n_ao;
         // Gamma of the block, with the Cartesian normalization ratios and 1/2 deg folded in
-        for (int c = lane; c < ncomp; c += GEN_T) {
+        for (int c = c_lo + lane; c < c_hi; c += GEN_T) {
             const int D = c % nD, C = (c / nD) % nC, B = (c / (nD * nC)) % nB, A = c / (nD * nC * nB);
             const size_t i = ia + A, j = ib + B, k = ic + C, l = id + D;
             auto P = [&](size_t r, size_t s) { return g.P[r + s * N]; };
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(GEN_T) void k_grad_eri(GradArgs g, int la, int lb, 
                 }
                 __syncthreads();
                 build_r<R>(X, Fs, M, px, py, pz);
-                for (int c = lane; c < ncomp; c += GEN_T) {
+                for (int c = c_lo + lane; c < c_hi; c += GEN_T) {
                     const R w = R(Gm[c]) * pref;
                     if (w == R(0)) continue;
                     const int D = c % nD, C = (c / nD) % nC, B = (c / (nD * nC)) % nB, A = c / (nD * nC * nB);
@@ -320,11 +324,12 @@ __global__ __launch_bounds__(GEN_T) void k_grad_eri(GradArgs g, int la, int lb, 
             for (int w = 1; w < GEN_T / 64; ++w)
                 for (int x = 0; x < 9; ++x) v[x] += red[w][x];
             const int aa = g.sh_atom[sa], ab = g.sh_atom[sb], ac = g.sh_atom[sc], ad = g.sh_atom[sd];
+            // the blocks of the other slices add into the same row
             for (int d = 0; d < 3; ++d) {
-                row[3 * aa + d] += v[d];
-                row[3 * ab + d] += v[3 + d];
-                row[3 * ac + d] += v[6 + d];
-                row[3 * ad + d] -= v[d] + v[3 + d] + v[6 + d];
+                atomicAdd(&row[3 * aa + d], v[d]);
+                atomicAdd(&row[3 * ab + d], v[3 + d]);
+                atomicAdd(&row[3 * ac + d], v[6 + d]);
+                atomicAdd(&row[3 * ad + d], -(v[d] + v[3 + d] + v[6 + d]));
             }
         }
         __syncthreads();  // Gm and red are rewritten by the next quartet
@@ -543,17 +548,21 @@ bool grad_class_has_fp32(int la, int lb, int lc, int ld) {
     return u.n > 0 && u.fp32;
 }
 
-void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, hipStream_t s) {
+void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, const hipStream_t* ss,
+                     int n_streams, int* next) {
     const int k = combo_of(pair_class(la, lb), pair_class(lc, ld));
     const GradUnits& u = grad_units(k);
     if (u.n > 0) {
-        for (int i = 0; i < u.n; ++i) u.fn[i](fp32, g, grid, s);
+        for (int i = 0; i < u.n; ++i) u.fn[i](fp32, g, grid, ss[(*next)++ % n_streams]);
         return;
     }
+    const hipStream_t s = ss[(*next)++ % n_streams];
+    const int ncomp = md::ncart(la) * md::ncart(lb) * md::ncart(lc) * md::ncart(ld);
+    const dim3 blocks(grid, std::min(4, (ncomp + GEN_T - 1) / GEN_T));  // component slices
     if (fp32)
-        hipLaunchKernelGGL(k_grad_eri<float>, dim3(grid), dim3(GEN_T), 0, s, g, la, lb, lc, ld);
+        hipLaunchKernelGGL(k_grad_eri<float>, blocks, dim3(GEN_T), 0, s, g, la, lb, lc, ld);
     else
-        hipLaunchKernelGGL(k_grad_eri<double>, dim3(grid), dim3(GEN_T), 0, s, g, la, lb, lc, ld);
+        hipLaunchKernelGGL(k_grad_eri<double>, blocks, dim3(GEN_T), 0, s, g, la, lb, lc, ld);
 }
 
 void launch_grad_one(const GradArgs& g, int grid, hipStream_t s) {

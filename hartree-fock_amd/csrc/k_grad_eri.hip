@@ -3,11 +3,11 @@
 //
 // Compiled per class with -DHFX_GRAD_COMBO=k (k as in k_eri.hip).  Up to la+lb+lc+ld = 3 one
 // thread takes a whole quartet.  Above that the whole quartet spills to scratch, so the bra
-// components are split into chunks of HFX_GRAD_CB: one kernel per chunk, each thread takes
-// one (quartet, chunk) and repeats the Boys values, R and the E tables for its chunk.  A
-// translation unit holds the chunks HFX_GRAD_CHUNK_BEGIN .. + HFX_GRAD_CHUNKS - 1 of its class
-// and launches them in order on the caller's stream; the classes not compiled here use the
-// generic kernel of k_grad.hip.  The chunked classes are compiled in FP64 only
+// components are split into chunks of HFX_GRAD_CB: each thread takes one (quartet, chunk) and
+// repeats the Boys values, R and the E tables for its chunk.  A translation unit holds the
+// chunks HFX_GRAD_CHUNK_BEGIN .. + HFX_GRAD_CHUNKS - 1 of its class in one kernel (blockIdx.y
+// picks the chunk); the units of a class run one after the other on the caller's stream.  The
+// classes not compiled here use the generic kernel of k_grad.hip.  The chunked classes are compiled in FP64 only
 // (-DHFX_GRAD_FP64_ONLY): a mixed-precision gradient runs their whole list, the quartets below
 // the cut included, through the FP64 kernels in one pass (a second, FP32 pass over the short
 // tail of each chunk took longer than the FP64 work it replaced), and the compile time halves.
@@ -97,20 +97,12 @@ __device__ __forceinline__ R ed(const Ecoef<EA, EB, R>& E, R twoa) {
     return v;
 }
 
-// KC: the class (K_COMBO; it names the kernel in profiles), C0: the chunk's first bra
-// component (components C0 .. C0 + CB - 1, those below NAB).
-template <class R, int KC, int C0>
-__global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_grad_eri_t(GradArgs g) {
+// One chunk: bra components C0 .. C0 + CB - 1 (those below NAB) of every quartet of the list,
+// one quartet per thread; results into dst (the block's LDS gradient row, or its row of g.part).
+template <class R, int C0>
+__device__ __forceinline__ void grad_chunk(const GradArgs& g, R* gam, double* dst) {
     constexpr int BT = block_threads(CB * NCD * (int)sizeof(R));
     using Ls = Lists<R>;
-    __shared__ R gam[CB * NCD * BT];
-    extern __shared__ double sgrad[];
-    const int n3 = g.n3;
-    const bool lds_rows = n3 <= GRAD_LDS_MAX;
-    if (lds_rows)
-        for (int x = threadIdx.x; x < n3; x += BT) sgrad[x] = 0.0;
-    __syncthreads();
-    double* row = g.part + (size_t)blockIdx.x * g.part_stride + 2 * n3;
     R* gm = gam + threadIdx.x;  // this thread's slots, BT apart
     const long long lo = g.range[0], hi = g.range[1];
     const int N = g.n_ao;
@@ -318,7 +310,6 @@ __global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_gr
             });
         }
         const int aa = g.sh_atom[sa], ab = g.sh_atom[sb], ac = g.sh_atom[sc], ad = g.sh_atom[sd];
-        double* dst = lds_rows ? sgrad : row;
         for (int d = 0; d < 3; ++d) {
             atomicAdd(&dst[3 * aa + d], gA[d]);
             atomicAdd(&dst[3 * ab + d], gP[d] - gA[d]);
@@ -326,34 +317,47 @@ __global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_gr
             atomicAdd(&dst[3 * ad + d], -(gP[d] + gC[d]));
         }
     }
+}
+
+// The unit's chunks in one launch: blockIdx.y picks the chunk, so the chunks of a class run
+// side by side (a class with few quartets fills the machine only that way).  KC (the class)
+// and CH (the unit's first chunk) name the kernel in profiles.  Blocks of different chunks
+// share the rows of g.part: the LDS rows are added with atomics.
+template <class R, int KC, int CH>
+__global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_grad_eri_t(GradArgs g) {
+    constexpr int BT = block_threads(CB * NCD * (int)sizeof(R));
+    __shared__ R gam[CB * NCD * BT];
+    extern __shared__ double sgrad[];
+    const int n3 = g.n3;
+    const bool lds_rows = n3 <= GRAD_LDS_MAX;
+    if (lds_rows)
+        for (int x = threadIdx.x; x < n3; x += BT) sgrad[x] = 0.0;
+    __syncthreads();
+    double* row = g.part + (size_t)blockIdx.x * g.part_stride + 2 * n3;
+    sfor<CH_COUNT>([&](auto i_) {
+        constexpr int i = decltype(i_)::value;
+        if (blockIdx.y == i) grad_chunk<R, (CH + i) * CB>(g, gam, lds_rows ? sgrad : row);
+    });
     if (lds_rows) {
         __syncthreads();
         for (int x = threadIdx.x; x < n3; x += BT)
-            if (sgrad[x] != 0.0) row[x] += sgrad[x];
-    }
-}
-
-// The chunks of this unit, one after the other on the caller's stream (every kernel adds into
-// the same per-block rows of g.part, so they must not overlap).
-template <int I>
-void launch_chunks(int fp32, const GradArgs& g, int grid, hipStream_t s, size_t lds) {
-    if constexpr (I < CH_COUNT) {
-        constexpr int c0 = (CH_BEGIN + I) * CB;
-#ifdef HFX_GRAD_FP64_ONLY
-        (void)fp32;
-#else
-        if (fp32)
-            hipLaunchKernelGGL((k_grad_eri_t<float, K_COMBO, c0>), dim3(grid), dim3(block_threads(CB * NCD * 4)), lds, s, g);
-        else
-#endif
-            hipLaunchKernelGGL((k_grad_eri_t<double, K_COMBO, c0>), dim3(grid), dim3(block_threads(CB * NCD * 8)), lds, s,
-                               g);
-        launch_chunks<I + 1>(fp32, g, grid, s, lds);
+            if (sgrad[x] != 0.0) atomicAdd(&row[x], sgrad[x]);
     }
 }
 
 void launch(int fp32, const GradArgs& g, int grid, hipStream_t s) {
-    launch_chunks<0>(fp32, g, grid, s, g.n3 <= GRAD_LDS_MAX ? (size_t)g.n3 * sizeof(double) : 0);
+    const size_t lds = g.n3 <= GRAD_LDS_MAX ? (size_t)g.n3 * sizeof(double) : 0;
+    const dim3 blocks(grid, CH_COUNT);
+#ifdef HFX_GRAD_FP64_ONLY
+    (void)fp32;
+#else
+    if (fp32)
+        hipLaunchKernelGGL((k_grad_eri_t<float, K_COMBO, CH_BEGIN>), blocks, dim3(block_threads(CB * NCD * 4)), lds, s,
+                           g);
+    else
+#endif
+        hipLaunchKernelGGL((k_grad_eri_t<double, K_COMBO, CH_BEGIN>), blocks, dim3(block_threads(CB * NCD * 8)), lds,
+                           s, g);
 }
 
 #ifdef HFX_GRAD_FP64_ONLY
