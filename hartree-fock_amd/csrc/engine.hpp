@@ -140,10 +140,11 @@ struct GradArgs {
     long long part_stride;
     int n3;                  // 3 * n_atoms
 };
-// The class's two-electron gradient kernels; each launch goes to ss[(*next)++ % n_streams]
-// (every kernel adds into g.part with atomics, so launches on different streams may overlap).
-void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, const hipStream_t* ss,
-                     int n_streams, int* next);
+// Launch units of a class's two-electron gradient: grad_class_units of them (translation units
+// of unrolled kernels), or one (the generic kernel) when that is 0.  Every kernel adds into
+// g.part with atomics, so launches on different streams may overlap.
+int grad_class_units(int la, int lb, int lc, int ld);
+void launch_grad_unit(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int unit, int grid, hipStream_t s);
 // The two-electron gradient kernels of one quartet class, compiled with every angular loop
 // unrolled (k_grad_eri.hip, one or more translation units per class, each for a range of
 // bra-component chunks starting at first_chunk; fp32: the unit has FP32 kernels too); classes

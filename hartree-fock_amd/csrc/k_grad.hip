@@ -548,15 +548,16 @@ bool grad_class_has_fp32(int la, int lb, int lc, int ld) {
     return u.n > 0 && u.fp32;
 }
 
-void launch_grad_eri(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int grid, const hipStream_t* ss,
-                     int n_streams, int* next) {
-    const int k = combo_of(pair_class(la, lb), pair_class(lc, ld));
-    const GradUnits& u = grad_units(k);
+int grad_class_units(int la, int lb, int lc, int ld) {
+    return grad_units(combo_of(pair_class(la, lb), pair_class(lc, ld))).n;
+}
+
+void launch_grad_unit(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int unit, int grid, hipStream_t s) {
+    const GradUnits& u = grad_units(combo_of(pair_class(la, lb), pair_class(lc, ld)));
     if (u.n > 0) {
-        for (int i = 0; i < u.n; ++i) u.fn[i](fp32, g, grid, ss[(*next)++ % n_streams]);
+        u.fn[unit](fp32, g, grid, s);
         return;
     }
-    const hipStream_t s = ss[(*next)++ % n_streams];
     const int ncomp = md::ncart(la) * md::ncart(lb) * md::ncart(lc) * md::ncart(ld);
     const dim3 blocks(grid, std::min(4, (ncomp + GEN_T - 1) / GEN_T));  // component slices
     if (fp32)
