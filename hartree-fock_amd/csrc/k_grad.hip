@@ -154,8 +154,11 @@ struct GradLists<float> {
 // ---------------------------------------------------------------------------------------
 constexpr int GEN_T = 256;  // threads of the generic two-electron kernel's block (four wavefronts)
 
+// At least four wavefronts per SIMD (128 VGPRs, a 160-byte spill in FP64): the kernel waits on
+// LDS, and more resident blocks hide it (dddp 13.2 ms at two per SIMD, 10.6 at three, 8.9 at
+// four, 8.5 at six with a 368-byte spill).
 template <class R>
-__global__ __launch_bounds__(GEN_T) void k_grad_eri(GradArgs g, int la, int lb, int lc, int ld) {
+__global__ __launch_bounds__(GEN_T, 4) void k_grad_eri(GradArgs g, int la, int lb, int lc, int ld) {
     using L = GradLists<R>;
     __shared__ R X[RD * RD * RD];
     __shared__ R Eb[3 * EI * EI * ET];
