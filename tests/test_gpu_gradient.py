@@ -88,6 +88,10 @@ def test_gradient_mixed_precision(engine_lib):
     assert np.abs(gmx - g64).max() <= 1e-6
     # the same screened quartets, every one with at least one primitive quartet
     assert (smx["class_prims"] == s64["class_prims"]).all() and s64["class_prims"].sum() >= s64["n_quartets"]
+    # per-class timing runs the class kernels one after another on one stream instead of spread
+    # over four: the same gradient up to the order of the atomic additions
+    gser, _, sser = lib.rhf_gradient(e, b, P, W, flags=lib.HFX_JK_CLASS_TIMING)
+    assert np.abs(gser - g64).max() <= 1e-11 and (sser["class_ms"] > 0).sum() > 10
 
 
 def test_autograd_function():
