@@ -140,15 +140,14 @@ struct GradArgs {
     long long part_stride;
     int n3;                  // 3 * n_atoms
 };
-// Launch units of a class's two-electron gradient: grad_class_units of them (translation units
-// of unrolled kernels), or one (the generic kernel) when that is 0.  Every kernel adds into
-// g.part with atomics, so launches on different streams may overlap.
+// Launch units of a class's two-electron gradient: grad_class_units of them (its unrolled
+// kernels, one per chunk of bra components), or one (the generic kernel) when that is 0.  Every
+// kernel adds into g.part with atomics, so launches on different streams may overlap.
 int grad_class_units(int la, int lb, int lc, int ld);
 void launch_grad_unit(int fp32, const GradArgs& g, int la, int lb, int lc, int ld, int unit, int grid, hipStream_t s);
 // The two-electron gradient kernels of one quartet class, compiled with every angular loop
-// unrolled (k_grad_eri.hip, one or more translation units per class, each for a range of
-// bra-component chunks starting at first_chunk; fp32: the unit has FP32 kernels too); classes
-// without one use the generic kernel of k_grad.hip.
+// unrolled (k_grad_eri.hip): one launcher per chunk of bra components (its index first_chunk;
+// fp32: it has an FP32 kernel too); classes without any use the generic kernel of k_grad.hip.
 using GradClassFn = void (*)(int fp32, const GradArgs& g, int grid, hipStream_t s);
 bool register_grad_class(int combo, int first_chunk, GradClassFn fn, bool fp32);
 // A mixed-precision gradient runs an FP32 pass for the class only where this holds (unrolled

@@ -514,9 +514,8 @@ __global__ void k_grad_reduce(const double* __restrict__ part, long long stride,
 
 }  // namespace
 
-// Launchers of the compile-time-unrolled classes (k_grad_eri.hip: one or more translation
-// units per class, each for a range of bra-component chunks), registered at load time and
-// called in the order of their first chunk.
+// Launchers of the compile-time-unrolled classes (k_grad_eri.hip: one per chunk of bra
+// components), registered at load time, kept in chunk order.
 namespace {
 constexpr int GRAD_UNITS_MAX = 64;
 struct GradUnits {

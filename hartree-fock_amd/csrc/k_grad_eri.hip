@@ -5,9 +5,9 @@
 // thread takes a whole quartet.  Above that the whole quartet spills to scratch, so the bra
 // components are split into chunks of HFX_GRAD_CB: each thread takes one (quartet, chunk) and
 // repeats the Boys values, R and the E tables for its chunk.  A translation unit holds the
-// chunks HFX_GRAD_CHUNK_BEGIN .. + HFX_GRAD_CHUNKS - 1 of its class in one kernel (blockIdx.y
-// picks the chunk); the units of a class run one after the other on the caller's stream.  The
-// classes not compiled here use the generic kernel of k_grad.hip.  The chunked classes are compiled in FP64 only
+// chunks HFX_GRAD_CHUNK_BEGIN .. + HFX_GRAD_CHUNKS - 1 of its class, one kernel each, and
+// registers each as a launch unit of the class.  The classes not compiled here use the generic
+// kernel of k_grad.hip.  The chunked classes are compiled in FP64 only
 // (-DHFX_GRAD_FP64_ONLY): a mixed-precision gradient runs their whole list, the quartets below
 // the cut included, through the FP64 kernels in one pass (a second, FP32 pass over the short
 // tail of each chunk took longer than the FP64 work it replaced), and the compile time halves.
@@ -319,11 +319,12 @@ __device__ __forceinline__ void grad_chunk(const GradArgs& g, R* gam, double* ds
     }
 }
 
-// The unit's chunks in one launch: blockIdx.y picks the chunk, so the chunks of a class run
-// side by side (a class with few quartets fills the machine only that way).  KC (the class)
-// and CH (the unit's first chunk) name the kernel in profiles.  Blocks of different chunks
-// share the rows of g.part: the LDS rows are added with atomics.
-template <class R, int KC, int CH>
+// One chunk per kernel (C0: its first bra component); KC (the class) names the kernel in
+// profiles.  The chunks of a class are separate launch units that the caller spreads over its
+// streams, so blocks of different kernels share the rows of g.part: the LDS rows are added with
+// atomics.  (One kernel for a unit's chunks, blockIdx.y picking the chunk, ran the same but
+// took half as long again to compile: the compiler allocates registers over all the bodies.)
+template <class R, int KC, int C0>
 __global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_grad_eri_t(GradArgs g) {
     constexpr int BT = block_threads(CB * NCD * (int)sizeof(R));
     __shared__ R gam[CB * NCD * BT];
@@ -334,10 +335,7 @@ __global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_gr
         for (int x = threadIdx.x; x < n3; x += BT) sgrad[x] = 0.0;
     __syncthreads();
     double* row = g.part + (size_t)blockIdx.x * g.part_stride + 2 * n3;
-    sfor<CH_COUNT>([&](auto i_) {
-        constexpr int i = decltype(i_)::value;
-        if (blockIdx.y == i) grad_chunk<R, (CH + i) * CB>(g, gam, lds_rows ? sgrad : row);
-    });
+    grad_chunk<R, C0>(g, gam, lds_rows ? sgrad : row);
     if (lds_rows) {
         __syncthreads();
         for (int x = threadIdx.x; x < n3; x += BT)
@@ -345,26 +343,35 @@ __global__ __launch_bounds__(block_threads(CB * NCD * (int)sizeof(R))) void k_gr
     }
 }
 
+// Chunk I of this unit (chunk CH_BEGIN + I of the class).
+template <int I>
 void launch(int fp32, const GradArgs& g, int grid, hipStream_t s) {
+    constexpr int c0 = (CH_BEGIN + I) * CB;
     const size_t lds = g.n3 <= GRAD_LDS_MAX ? (size_t)g.n3 * sizeof(double) : 0;
-    const dim3 blocks(grid, CH_COUNT);
 #ifdef HFX_GRAD_FP64_ONLY
     (void)fp32;
 #else
     if (fp32)
-        hipLaunchKernelGGL((k_grad_eri_t<float, K_COMBO, CH_BEGIN>), blocks, dim3(block_threads(CB * NCD * 4)), lds, s,
-                           g);
+        hipLaunchKernelGGL((k_grad_eri_t<float, K_COMBO, c0>), dim3(grid), dim3(block_threads(CB * NCD * 4)), lds, s, g);
     else
 #endif
-        hipLaunchKernelGGL((k_grad_eri_t<double, K_COMBO, CH_BEGIN>), blocks, dim3(block_threads(CB * NCD * 8)), lds,
-                           s, g);
+        hipLaunchKernelGGL((k_grad_eri_t<double, K_COMBO, c0>), dim3(grid), dim3(block_threads(CB * NCD * 8)), lds, s,
+                           g);
 }
 
 #ifdef HFX_GRAD_FP64_ONLY
-const bool registered = register_grad_class(K_COMBO, CH_BEGIN, &launch, false);
+constexpr bool HAS_FP32 = false;
 #else
-const bool registered = register_grad_class(K_COMBO, CH_BEGIN, &launch, true);
+constexpr bool HAS_FP32 = true;
 #endif
+template <int I>
+bool register_chunks() {
+    if constexpr (I < CH_COUNT)
+        return register_grad_class(K_COMBO, CH_BEGIN + I, &launch<I>, HAS_FP32) && register_chunks<I + 1>();
+    else
+        return true;
+}
+const bool registered = register_chunks<0>();
 
 }  // namespace
 }  // namespace hfx
