@@ -280,8 +280,8 @@ def roofline_of(stmx, cm64, cm32):
     """Roofline of the dominant class kernel group (one class, one precision) of a build.
 
     `achieved` counts the model FLOPs of the primitive quartets the kernel executes (the engine
-    prunes primitive pairs below 1e-20 / 1e-11 in FP64 / FP32, which the unpruned model count of
-    the survey would still credit); the kernels are VALU-bound (no MFMA), so the bound is the
+    prunes primitive pairs below 1e-20 / 1e-11 in FP64 / FP32, which an unpruned model count
+    would still credit); the kernels are VALU-bound (no MFMA), so the bound is the
     vector pipe, priced at MI355X's listed vector peak (FP32 157.3 TFLOP/s counts packed FP32,
     which these kernels do not use: their non-packed ceiling is 78.6, given as peak_nonpacked)."""
     groups = [(cm64[k], "f64", k) for k in range(21)] + [(cm32[k], "f32", k) for k in range(21)]
