@@ -546,15 +546,19 @@ __device__ __forceinline__ void run_chunk(const EriArgs& a, int vb, int nvb, R* 
             const long long base = c0 + (it + rot < nit ? it + rot : it + rot - nit) * blockDim.x;
             if (tile) {
                 // The rows follow the bra pair of the iteration's first entry once it holds
-                // at least a quarter of the iteration.
+                // at least a quarter of the iteration.  pid0 and tile_pid are the same in every
+                // thread, so while the rows already hold pid0 the block skips the count and its
+                // barrier (the rows only take atomic adds until the next flush).
                 const int pid0 = (int)a.list[lo + base].x;
-                const long long w = base + threadIdx.x;
-                const int mine = (w < c1 && (int)a.list[lo + w].x == pid0) ? 1 : 0;
-                const int cnt = __syncthreads_count(mine);
-                if (pid0 != tile_pid && cnt >= (int)blockDim.x / 4) {
-                    if (tile_pid >= 0) flush_rows(a, tile, tile_pid, n_tiles);
-                    __syncthreads();
-                    tile_pid = pid0;
+                if (pid0 != tile_pid) {
+                    const long long w = base + threadIdx.x;
+                    const int mine = (w < c1 && (int)a.list[lo + w].x == pid0) ? 1 : 0;
+                    const int cnt = __syncthreads_count(mine);
+                    if (cnt >= (int)blockDim.x / 4) {
+                        if (tile_pid >= 0) flush_rows(a, tile, tile_pid, n_tiles);
+                        __syncthreads();
+                        tile_pid = pid0;
+                    }
                 }
             }
             const long long w = base + threadIdx.x;
